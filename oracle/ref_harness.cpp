@@ -1,0 +1,119 @@
+// Harness that drives the reference's own CSB branch (include/xsm/DEXHVcsb.cxx:60-71)
+// without its Eigen-only dense branch.  Built by oracle/Makefile against the reference
+// headers where they lie; output goes to oracle/_ref/ only.  Test infrastructure: it is
+// the checker / CPU baseline, never the product.
+//
+// Modes:
+//   dumpint <sys.xml> <out.bin>            m-scheme ESP and VSPX tensors after xReadSPInt
+//   csb     <sys.xml> <out.EE> <bloc> <iters> <nstate>
+//           builds SparseMatrixCSB via PPMatrix2CSBBlk, times it, runs the reference
+//           Davidson on it (DavidsonDiagonalization), prints timings as JSON on stdout
+//   spmm    <sys.xml> <m> <reps> <out.bin>
+//           builds CSB, applies TimesMatrix to m deterministic vectors reps times,
+//           writes n, m, X, Y and prints the mean time per call as JSON
+#include <cstdint>
+typedef uint16_t spsint;
+typedef unsigned long long uint_nbasis;
+#include <av/argparser.cxx>
+#include <av/statusclass.cxx>
+#include <xsm/xManyBodyStates.cxx>
+#include <xsm/xsphericalint.cxx>
+#include <sm/MakeSPO.cxx>
+#include <sm/spherical2sp.cxx>
+#include <tav/sparse.cxx>
+#include <tav/csbmatrix.cxx>
+#include <xsm/xReadSPInt.cxx>
+#include <sm/PPMatrix.cxx>
+#include <kdav/davidson.cxx>
+#include <chrono>
+#include <cstdio>
+#include <vector>
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Setup {
+  SM::SingleParticleStates sps;
+  SM::Many_Body_States st;
+  tav::STensor<spsint, double> ESP{2};
+  tav::STensor<spsint, double> VSPX{4};
+  XMLNode main, sys;
+};
+
+static void load(Setup &s, const char *xml) {
+  s.main = XMLNode::openFileHelper(xml, xsm::cosmo);
+  SM::Make(s.sps, s.main.getChildNode(xsm::valencespace));
+  s.sys = xml::SelectNode(s.main, xsm::system, "name");
+  SM::Get(s.st, s.main, s.sys);
+  xReadSPInt(s.ESP, s.VSPX, s.sys, s.main, s.sps);
+}
+
+static void dump_tensor(FILE *f, tav::STensor<spsint, double> &T, uint32_t rank) {
+  uint64_t cnt = T.size();
+  fwrite(&rank, 4, 1, f);
+  fwrite(&cnt, 8, 1, f);
+  for (auto it = T.begin(); it != T.end(); ++it) {
+    for (uint32_t r = 0; r < rank; ++r) { uint16_t k = it->first[r]; fwrite(&k, 2, 1, f); }
+    double v = it->second;
+    fwrite(&v, 8, 1, f);
+  }
+}
+
+int main(int argc, char **argv) {
+  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|spmm <sys.xml> ...\n"); return 2; }
+  std::string mode = argv[1];
+  Setup s;
+  load(s, argv[2]);
+  if (mode == "dumpint") {
+    FILE *f = fopen(argv[3], "wb");
+    dump_tensor(f, s.ESP, 2);
+    dump_tensor(f, s.VSPX, 4);
+    fclose(f);
+    printf("{\"n\": %llu, \"esp\": %zu, \"vspx\": %zu}\n", (unsigned long long)s.st.n, s.ESP.size(), s.VSPX.size());
+    return 0;
+  }
+  SM::PPMatrix HH(s.st);
+  HH.push_back(&s.ESP);
+  HH.push_back(&s.VSPX);
+  uint16_t br = std::sqrt(1. * s.st.n) + 1;
+  tav::SparseMatrixCSB<double, uint32_t, uint16_t> FHH(s.st.n, br, br, true);
+  double t0 = now_s();
+  SM::PPMatrix2CSBBlk(FHH, HH);
+  double t_build = now_s() - t0;
+  uint64_t nnz = 0;
+  for (int i = 0; i < FHH.numBlockRow; ++i)
+    for (int j = 0; j < FHH.numBlockCol; ++j) nnz += FHH.Blocks[i][j].nnz;
+  if (mode == "csb") {
+    int bloc = atoi(argv[4]), iters = atoi(argv[5]), nstate = atoi(argv[6]);
+    if (bloc < nstate) bloc = nstate;
+    t0 = now_s();
+    DavidsonDiagonalization(argv[3], FHH, bloc, iters, nstate);
+    double t_dav = now_s() - t0;
+    printf("{\"n\": %llu, \"nnz_upper\": %llu, \"block_rows\": %d, \"t_build\": %.6f, \"t_davidson\": %.6f}\n",
+           (unsigned long long)s.st.n, (unsigned long long)nnz, (int)br, t_build, t_dav);
+    return 0;
+  }
+  if (mode == "spmm") {
+    int m = atoi(argv[3]), reps = atoi(argv[4]);
+    uint64_t n = s.st.n;
+    std::vector<double> X(n * m), Y(n * m);
+    // deterministic input: x[v][i] = sin(0.37*i + 1.3*v) (both sides compute the same doubles)
+    for (int v = 0; v < m; ++v)
+      for (uint64_t i = 0; i < n; ++i) X[v * n + i] = std::sin(0.37 * double(i) + 1.3 * double(v));
+    FHH.TimesMatrix(Y.data(), X.data(), m);  // warm-up
+    t0 = now_s();
+    for (int r = 0; r < reps; ++r) FHH.TimesMatrix(Y.data(), X.data(), m);
+    double t_mv = (now_s() - t0) / reps;
+    FILE *f = fopen(argv[5], "wb");
+    uint64_t mm = m;
+    fwrite(&n, 8, 1, f); fwrite(&mm, 8, 1, f);
+    fwrite(X.data(), 8, n * m, f); fwrite(Y.data(), 8, n * m, f);
+    fclose(f);
+    printf("{\"n\": %llu, \"nnz_upper\": %llu, \"block_rows\": %d, \"t_build\": %.6f, \"t_spmm\": %.6f, \"m\": %d}\n",
+           (unsigned long long)n, (unsigned long long)nnz, (int)br, t_build, t_mv, m);
+    return 0;
+  }
+  fprintf(stderr, "unknown mode %s\n", mode.c_str());
+  return 2;
+}
