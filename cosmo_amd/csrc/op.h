@@ -1,0 +1,42 @@
+// op.h -- the device-resident operator object behind cosmo_op.
+#pragma once
+#include "device.h"
+
+namespace cosmo {
+
+// full symmetric rows (both triangles), columns ascending within each row
+struct CsrDev {
+  int64_t n = 0, nnz = 0, nnz_upper = 0;
+  int64_t *rowptr = nullptr;
+  uint32_t *cols = nullptr;
+  double *v64 = nullptr;
+  float *v32 = nullptr;
+  double *diag = nullptr;
+};
+
+struct Op {
+  int device = 0;
+  int64_t n = 0;
+  bool is_virtual = false;
+  int flags = 0;
+  CsrDev csr;
+  DevTablesOwner tab;          // builder tables (kept for the virtual apply)
+  std::vector<double> hdiag;   // host copy of the diagonal (MatrixBase::Diagonal)
+  // staging for the host-pointer apply
+  void *sx = nullptr, *sy = nullptr, *sx32 = nullptr, *sy32 = nullptr;
+  size_t s_elems = 0;
+  double build_ms = 0, last_apply_ms = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+int build_stored(Op &op, const System &s, int flags, hipStream_t st);
+int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int prec, hipStream_t st);
+int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
+int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
+int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);
+int set_device_impl(int d);
+int current_device();
+
+}  // namespace cosmo
+
+struct cosmo_op_s : cosmo::Op {};

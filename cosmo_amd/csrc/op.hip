@@ -1,0 +1,383 @@
+// op.hip -- C ABI of the device-resident Hamiltonian operator (cosmo_op_*).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "op.h"
+
+namespace cosmo {
+int virtual_diag(const DevTables &T, int64_t n, double *d_diag, hipStream_t st);
+
+static void free_op(Op *op) {
+  if (!op) return;
+  hipSetDevice(op->device);
+  CsrDev &M = op->csr;
+  hipFree(M.rowptr);
+  hipFree(M.cols);
+  hipFree(M.v64);
+  hipFree(M.v32);
+  hipFree(M.diag);
+  free_tables(op->tab);
+  hipFree(op->sx);
+  hipFree(op->sy);
+  hipFree(op->sx32);
+  hipFree(op->sy32);
+  if (op->e0) hipEventDestroy(op->e0);
+  if (op->e1) hipEventDestroy(op->e1);
+}
+
+static int fetch_diag(Op &op, hipStream_t st) {
+  op.hdiag.assign(op.n, 0.0);
+  if (op.n == 0) return COSMO_OK;
+  HIPCHK(hipMemcpyAsync(op.hdiag.data(), op.csr.diag, sizeof(double) * op.n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return COSMO_OK;
+}
+
+static int ensure_staging(Op &op, size_t elems) {
+  if (elems <= op.s_elems) return COSMO_OK;
+  hipFree(op.sx);
+  hipFree(op.sy);
+  hipFree(op.sx32);
+  hipFree(op.sy32);
+  op.sx = op.sy = op.sx32 = op.sy32 = nullptr;
+  HIPCHK(hipMalloc(&op.sx, elems * sizeof(double)));
+  HIPCHK(hipMalloc(&op.sy, elems * sizeof(double)));
+  HIPCHK(hipMalloc(&op.sx32, elems * sizeof(float)));
+  HIPCHK(hipMalloc(&op.sy32, elems * sizeof(float)));
+  op.s_elems = elems;
+  return COSMO_OK;
+}
+
+// Upper-triangle rows (cols ascending per row) -> full symmetric rows on the device.
+static int upload_upper(Op &op, int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                        int flags, hipStream_t st) {
+  const bool keep = flags & COSMO_OP_KEEP_SMALL;
+  std::vector<int64_t> cnt(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      int64_t c = cols[e];
+      if (c < r || c >= n) return fail(COSMO_EINVAL, "upper rows must hold columns in [row, n)");
+      if (!keep && c != r && std::fabs(vals[e]) < 1e-14) continue;
+      cnt[r + 1]++;
+      if (c != r) cnt[c + 1]++;
+    }
+  for (int64_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+  const int64_t nnz = cnt[n];
+  std::vector<uint32_t> hc(std::max<int64_t>(nnz, 1));
+  std::vector<double> hv(std::max<int64_t>(nnz, 1));
+  std::vector<double> hd(n, 0.0);
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  // lower entries of row c come from rows r < c, visited in ascending r: they land first,
+  // in ascending column order; then the row's own upper entries, also ascending
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      int64_t c = cols[e];
+      if (!keep && c != r && std::fabs(vals[e]) < 1e-14) continue;
+      if (c == r) hd[r] = vals[e];
+      if (c != r) {
+        hc[fill[c]] = (uint32_t)r;
+        hv[fill[c]++] = vals[e];
+      }
+    }
+  // the upper parts go after the lower parts of each row
+  std::vector<int64_t> up(n, 0);
+  for (int64_t r = 0; r < n; ++r) up[r] = fill[r];
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+      int64_t c = cols[e];
+      if (!keep && c != r && std::fabs(vals[e]) < 1e-14) continue;
+      hc[up[r]] = (uint32_t)c;
+      hv[up[r]++] = vals[e];
+    }
+  CsrDev &M = op.csr;
+  M.n = n;
+  M.nnz = nnz;
+  int64_t nup = 0;
+  for (int64_t r = 0; r < n; ++r) nup += up[r] - fill[r];
+  M.nnz_upper = nup;
+  HIPCHK(hipMalloc(&M.rowptr, sizeof(int64_t) * (n + 1)));
+  HIPCHK(hipMalloc(&M.cols, sizeof(uint32_t) * hc.size()));
+  HIPCHK(hipMalloc(&M.v64, sizeof(double) * hv.size()));
+  HIPCHK(hipMalloc(&M.diag, sizeof(double) * std::max<int64_t>(n, 1)));
+  HIPCHK(hipMemcpyAsync(M.rowptr, cnt.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(M.cols, hc.data(), sizeof(uint32_t) * hc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(M.v64, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
+  if (n) HIPCHK(hipMemcpyAsync(M.diag, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  if (!(flags & COSMO_OP_NO_FP32)) {
+    std::vector<float> hf(hv.size());
+    for (size_t i = 0; i < hv.size(); ++i) hf[i] = (float)hv[i];
+    HIPCHK(hipMalloc(&M.v32, sizeof(float) * hf.size()));
+    HIPCHK(hipMemcpyAsync(M.v32, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  op.hdiag = hd;
+  return COSMO_OK;
+}
+
+}  // namespace cosmo
+
+using namespace cosmo;
+
+extern "C" {
+
+int cosmo_set_device(int device) { return set_device_impl(device); }
+
+int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
+  if (!s || !out) return fail(COSMO_EINVAL, "cosmo_op_create: null argument");
+  if (s->nv + s->zv > 24) return fail(COSMO_EUNSUP, "cosmo_op_create: more than 24 valence particles");
+  if (s->dim >= (int64_t)0xffffffffLL) return fail(COSMO_EUNSUP, "cosmo_op_create: dimension beyond 2^32-1");
+  auto *op = new cosmo_op_s();
+  op->device = current_device();
+  op->n = s->dim;
+  op->flags = flags;
+  op->is_virtual = (flags & COSMO_OP_VIRTUAL) != 0;
+  hipStream_t st = lib_stream();
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = upload_tables(*s, op->tab, st);
+  if (!rc) {
+    if (op->is_virtual) {
+      op->csr.n = op->n;
+      if (hipMalloc(&op->csr.diag, sizeof(double) * std::max<int64_t>(op->n, 1)) != hipSuccess)
+        rc = fail(COSMO_ENOMEM, "diagonal allocation failed");
+      else
+        rc = virtual_diag(op->tab.t, op->n, op->csr.diag, st);
+    } else {
+      rc = build_stored(*op, *s, flags, st);
+    }
+  }
+  if (!rc) rc = fetch_diag(*op, st);
+  if (rc) {
+    free_op(op);
+    delete op;
+    return rc;
+  }
+  op->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  hipEventCreate(&op->e0);
+  hipEventCreate(&op->e1);
+  *out = op;
+  return COSMO_OK;
+}
+
+int cosmo_op_create_upper(int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals, int flags,
+                          cosmo_op *out) {
+  if (!out || n < 0 || (n && (!rowptr || !cols || !vals))) return fail(COSMO_EINVAL, "cosmo_op_create_upper: bad args");
+  if (n >= (int64_t)0xffffffffLL) return fail(COSMO_EUNSUP, "dimension beyond 2^32-1");
+  auto *op = new cosmo_op_s();
+  op->device = current_device();
+  op->n = n;
+  op->flags = flags;
+  hipStream_t st = lib_stream();
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = upload_upper(*op, n, rowptr, cols, vals, flags, st);
+  if (rc) {
+    free_op(op);
+    delete op;
+    return rc;
+  }
+  op->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  hipEventCreate(&op->e0);
+  hipEventCreate(&op->e1);
+  *out = op;
+  return COSMO_OK;
+}
+
+int cosmo_op_read_hh(const char *path, int flags, cosmo_op *out) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(COSMO_EIO, std::string("File ") + path + " is not found");
+  uint64_t n = 0;
+  if (fread(&n, 8, 1, f) != 1) {
+    fclose(f);
+    return fail(COSMO_EIO, "truncated .HH header");
+  }
+  std::vector<int64_t> rp(n + 1, 0), cols;
+  std::vector<double> vals;
+  std::vector<uint64_t> tmp;
+  for (uint64_t r = 0; r < n; ++r) {
+    uint32_t nz = 0;
+    if (fread(&nz, 4, 1, f) != 1) {
+      fclose(f);
+      return fail(COSMO_EIO, "truncated .HH row header");
+    }
+    tmp.resize(nz);
+    size_t old = vals.size();
+    vals.resize(old + nz);
+    if (fread(tmp.data(), 8, nz, f) != nz || fread(vals.data() + old, 8, nz, f) != nz) {
+      fclose(f);
+      return fail(COSMO_EIO, "truncated .HH row");
+    }
+    for (uint32_t i = 0; i < nz; ++i) cols.push_back((int64_t)tmp[i]);
+    rp[r + 1] = rp[r] + nz;
+  }
+  fclose(f);
+  return cosmo_op_create_upper((int64_t)n, rp.data(), cols.data(), vals.data(), flags | COSMO_OP_KEEP_SMALL, out);
+}
+
+int cosmo_op_destroy(cosmo_op op) {
+  free_op(op);
+  delete op;
+  return COSMO_OK;
+}
+
+int64_t cosmo_op_size(cosmo_op op) { return op ? op->n : COSMO_EINVAL; }
+
+int cosmo_op_nnz(cosmo_op op, int64_t *nnz_upper, int64_t *nnz_full) {
+  if (!op) return fail(COSMO_EINVAL, "null operator");
+  if (nnz_upper) *nnz_upper = op->csr.nnz_upper;
+  if (nnz_full) *nnz_full = op->csr.nnz;
+  return COSMO_OK;
+}
+
+int64_t cosmo_op_bytes(cosmo_op op) {
+  if (!op) return COSMO_EINVAL;
+  const CsrDev &M = op->csr;
+  int64_t b = (int64_t)op->tab.bytes + 8 * M.n;
+  if (!op->is_virtual) b += 8 * (M.n + 1) + M.nnz * (4 + 8 + (M.v32 ? 4 : 0));
+  return b;
+}
+
+int cosmo_op_diagonal(cosmo_op op, double *d) {
+  if (!op || !d) return fail(COSMO_EINVAL, "null argument");
+  std::copy(op->hdiag.begin(), op->hdiag.end(), d);
+  return COSMO_OK;
+}
+
+int cosmo_op_apply_device(cosmo_op op, const void *X, void *Y, int m, int ld, int precision, void *stream) {
+  if (!op || (!X && m) || (!Y && m) || m < 0 || ld < m) return fail(COSMO_EINVAL, "cosmo_op_apply_device: bad args");
+  hipSetDevice(op->device);
+  hipStream_t st = stream ? (hipStream_t)stream : lib_stream();
+  hipEventRecord(op->e0, st);
+  int rc = op_apply_dev(*op, X, Y, m, ld, ld, precision, st);
+  hipEventRecord(op->e1, st);
+  return rc;
+}
+
+int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m, int precision) {
+  if (!op || m < 0 || (m && (!x || !y))) return fail(COSMO_EINVAL, "cosmo_op_apply: bad args");
+  if (m == 0) return COSMO_OK;
+  hipSetDevice(op->device);
+  hipStream_t st = lib_stream();
+  const int64_t n = op->n;
+  int rc = ensure_staging(*op, (size_t)n * m);
+  if (rc) return rc;
+  std::vector<double> h((size_t)n * m);
+  for (int v = 0; v < m; ++v)
+    for (int64_t r = 0; r < n; ++r) h[(size_t)r * m + v] = x[v][r];
+  HIPCHK(hipMemcpyAsync(op->sx, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+  if (precision == COSMO_PREC_FP32) {
+    rc = cast_pack(n, m, (const double *)op->sx, m, (float *)op->sx32, m, st);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(op->e0, st));
+    rc = op_apply_dev(*op, op->sx32, op->sy32, m, m, m, precision, st);
+    HIPCHK(hipEventRecord(op->e1, st));
+    if (rc) return rc;
+    rc = cast_unpack(n, m, (const float *)op->sy32, m, (double *)op->sy, m, st);
+  } else {
+    HIPCHK(hipEventRecord(op->e0, st));
+    rc = op_apply_dev(*op, op->sx, op->sy, m, m, m, precision, st);
+    HIPCHK(hipEventRecord(op->e1, st));
+  }
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h.data(), op->sy, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0;
+  hipEventElapsedTime(&ms, op->e0, op->e1);
+  op->last_apply_ms = ms;
+  for (int v = 0; v < m; ++v)
+    for (int64_t r = 0; r < n; ++r) y[v][r] = h[(size_t)r * m + v];
+  return COSMO_OK;
+}
+
+int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, const int64_t *rowptr, int64_t *cols,
+                        double *vals) {
+  if (!op || !counts || r0 < 0 || r1 > op->n || r1 < r0) return fail(COSMO_EINVAL, "cosmo_op_upper_rows: bad args");
+  if (op->is_virtual) return fail(COSMO_EUNSUP, "virtual operator has no stored rows");
+  hipSetDevice(op->device);
+  hipStream_t st = lib_stream();
+  const CsrDev &M = op->csr;
+  std::vector<int64_t> rp(r1 - r0 + 1);
+  if (r1 == r0) return COSMO_OK;
+  HIPCHK(hipMemcpyAsync(rp.data(), M.rowptr + r0, sizeof(int64_t) * rp.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t e0 = rp[0], e1 = rp.back();
+  std::vector<uint32_t> hc(e1 - e0);
+  std::vector<double> hv(e1 - e0);
+  if (e1 > e0) {
+    HIPCHK(hipMemcpyAsync(hc.data(), M.cols + e0, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hv.data(), M.v64 + e0, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  for (int64_t r = r0; r < r1; ++r) {
+    int64_t c = 0;
+    int64_t o = rowptr ? rowptr[r - r0] : 0;
+    for (int64_t e = rp[r - r0]; e < rp[r - r0 + 1]; ++e) {
+      uint32_t col = hc[e - e0];
+      if ((int64_t)col < r) continue;
+      if (cols) {
+        cols[o + c] = col;
+        vals[o + c] = hv[e - e0];
+      }
+      ++c;
+    }
+    counts[r - r0] = c;
+  }
+  return COSMO_OK;
+}
+
+int cosmo_op_write_hh(cosmo_op op, const char *path) {
+  if (!op || !path) return fail(COSMO_EINVAL, "null argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(COSMO_EIO, std::string("cannot write ") + path);
+  uint64_t n = (uint64_t)op->n;
+  fwrite(&n, 8, 1, f);
+  const int64_t chunk = 1 << 16;
+  std::vector<int64_t> cnt, rp, cols;
+  std::vector<double> vals;
+  std::vector<uint64_t> c64;
+  for (int64_t r0 = 0; r0 < op->n; r0 += chunk) {
+    int64_t r1 = std::min<int64_t>(op->n, r0 + chunk);
+    cnt.assign(r1 - r0, 0);
+    int rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), nullptr, nullptr, nullptr);
+    if (rc) {
+      fclose(f);
+      return rc;
+    }
+    rp.assign(r1 - r0 + 1, 0);
+    for (int64_t i = 0; i < r1 - r0; ++i) rp[i + 1] = rp[i] + cnt[i];
+    cols.resize(rp.back());
+    vals.resize(rp.back());
+    rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), rp.data(), cols.data(), vals.data());
+    if (rc) {
+      fclose(f);
+      return rc;
+    }
+    for (int64_t i = 0; i < r1 - r0; ++i) {
+      uint32_t nz = (uint32_t)cnt[i];
+      fwrite(&nz, 4, 1, f);
+      c64.assign(cols.begin() + rp[i], cols.begin() + rp[i + 1]);
+      fwrite(c64.data(), 8, nz, f);
+      fwrite(vals.data() + rp[i], 8, nz, f);
+    }
+  }
+  if (fclose(f)) return fail(COSMO_EIO, std::string("write failed: ") + path);
+  return COSMO_OK;
+}
+
+int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms) {
+  if (!op) return fail(COSMO_EINVAL, "null operator");
+  if (build_ms) *build_ms = op->build_ms;
+  if (last_apply_ms) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, op->e0, op->e1) == hipSuccess) op->last_apply_ms = ms;
+    *last_apply_ms = op->last_apply_ms;
+  }
+  return COSMO_OK;
+}
+
+}  // extern "C"

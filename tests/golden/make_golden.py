@@ -1,0 +1,118 @@
+"""Writes the golden fixtures of tests/golden from the reference's own programs.
+
+Run in a container that has the CoSMo source tree at /root/reference (never on the GPU
+box): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
+into oracle/_ref from those sources, then this script runs them on seeded synthetic
+systems and stores inputs and outputs as small data files (npz / json / text).
+"""
+import hashlib
+import json
+import os
+import shutil
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+from cosmo_amd import synth  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# name, shell, Nv, Zv, 2Jz, parity, seed, full (commit .HH rows) , davidson block
+CASES = [
+    ("sd2p2n_s11", "sd", 2, 2, 0, "+", 11, True, 10),
+    ("sd3p1n_m1_s7", "sd", 1, 3, 2, "+", 7, True, 6),
+    ("fp2p2n_s5", "fp", 2, 2, 0, "+", 5, True, 10),
+    ("fp3p2n_m1_s3", "fp", 2, 3, 1, "-", 3, False, 8),
+    ("sd4p4n_s0", "sd", 4, 4, 0, "+", 0, False, 10),
+]
+
+
+def run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, stdin=subprocess.DEVNULL, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s failed: %s" % (cmd, r.stderr[-2000:]))
+    return r.stdout
+
+
+def read_dump(path):
+    data = open(path, "rb").read()
+    off = 0
+    out = []
+    for _ in range(2):
+        r, = struct.unpack_from("<I", data, off)
+        c, = struct.unpack_from("<Q", data, off + 4)
+        off += 12
+        keys = np.zeros((c, r), np.uint16)
+        vals = np.zeros(c)
+        for i in range(c):
+            keys[i] = struct.unpack_from("<%dH" % r, data, off)
+            off += 2 * r
+            vals[i], = struct.unpack_from("<d", data, off)
+            off += 8
+        out.append((keys, vals))
+    return out
+
+
+def last_json(text):
+    return json.loads([ln for ln in text.splitlines() if ln.startswith("{")][-1])
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def main():
+    for (name, shell, nv, zv, twojz, par, seed, full, bloc) in CASES:
+        d = tempfile.mkdtemp()
+        try:
+            xml = synth.write_system(d, shell, name, nv, zv, twojz, par, seed)
+            shutil.copy(os.path.join(d, name + ".int"), os.path.join(OUT, name + ".int"))
+            shutil.copy(xml, os.path.join(OUT, name + ".xml"))
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            run([os.path.join(REF, "ref_harness"), "dumpint", name + ".xml", "dump.bin"], d)
+            (ek, ev), (vk, vv) = read_dump(os.path.join(d, "dump.bin"))
+            run([os.path.join(REF, "XHH+JJ"), name + ".xml"], d)
+            info = last_json(run([os.path.join(REF, "ref_harness"), "spmm", name + ".xml", "3", "1", "spmm.bin"], d))
+            raw = open(os.path.join(d, "spmm.bin"), "rb").read()
+            n, m = struct.unpack_from("<QQ", raw, 0)
+            X = np.frombuffer(raw, np.float64, n * m, 16).reshape(m, n).T
+            Y = np.frombuffer(raw, np.float64, n * m, 16 + 8 * n * m).reshape(m, n).T
+            dinfo = last_json(run([os.path.join(REF, "ref_harness"), "csb", name + ".xml", "ref.EE", str(bloc), "0",
+                                    "0"], d))
+            E, dE, R, V = oracle.read_ee(os.path.join(d, "ref.EE"))
+            mbs = oracle.read_mbs(os.path.join(d, name + ".mbs"))
+            meta = dict(name=name, shell=shell, nv=nv, zv=zv, twojz=twojz, parity=par, seed=seed, n=int(n),
+                        nnz_upper_csb=int(info["nnz_upper"]), block_rows=int(info["block_rows"]),
+                        mbs_sha256=sha(os.path.join(d, name + ".mbs")), hh_sha256=sha(os.path.join(d, name + ".HH")),
+                        davidson_block=bloc, ref_evals=E.tolist(), ref_dE=dE.tolist(), ref_resid=R.tolist())
+            rp, cols, vals = oracle.read_hh(os.path.join(d, name + ".HH"))
+            meta["hh_nnz"] = int(rp[-1])
+            arrays = dict(esp_keys=ek, esp_vals=ev, vspx_keys=vk, vspx_vals=vv)
+            if full:
+                arrays.update(mbs=mbs, hh_rowptr=rp, hh_cols=cols, hh_vals=vals, spmm_x=X, spmm_y=Y)
+            else:
+                rng = np.random.default_rng(1)
+                rows = np.sort(rng.choice(n, 16, replace=False))
+                arrays.update(sample_rows=rows,
+                              sample_rowptr=np.concatenate([[0], np.cumsum(rp[rows + 1] - rp[rows])]),
+                              sample_cols=np.concatenate([cols[rp[r]:rp[r + 1]] for r in rows]),
+                              sample_vals=np.concatenate([vals[rp[r]:rp[r + 1]] for r in rows]),
+                              spmm_y_sample=Y[rows])
+                meta["spmm_y_sha256"] = hashlib.sha256(np.ascontiguousarray(Y.T).tobytes()).hexdigest()
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
+            with open(os.path.join(OUT, name + ".json"), "w") as f:
+                json.dump(meta, f, indent=1)
+            print(name, n, info["nnz_upper"], E[:3])
+        finally:
+            shutil.rmtree(d)
+
+
+if __name__ == "__main__":
+    main()

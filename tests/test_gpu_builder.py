@@ -1,0 +1,96 @@
+"""GPU Hamiltonian builder against the oracle (same m-scheme tables, so values are
+bit-exact) and against the reference's .HH rows (structure exact, values to CG noise)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import FULL_CASES, GOLDEN_CASES
+
+pytestmark = pytest.mark.gpu
+
+
+def full_rows_from_upper(n, rp, cols, vals):
+    """full symmetric rows (cols ascending) from upper rows"""
+    rows = [[] for _ in range(n)]
+    for r in range(n):
+        for e in range(rp[r], rp[r + 1]):
+            rows[r].append((cols[e], vals[e]))
+            if cols[e] != r:
+                rows[cols[e]].append((r, vals[e]))
+    return [sorted(x) for x in rows]
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_upper_rows_bit_exact_vs_oracle(gpu, golden, name):
+    g = golden(name)
+    s = gpu.System.from_xml(g.xml)
+    o = oracle.System(g.xml)
+    for keep in (False, True):
+        op = gpu.Operator.build(s, gpu.OP_KEEP_SMALL if keep else gpu.OP_STORED)
+        rp, c, v = op.upper_rows()
+        orp, oc, ov = o.h_upper(drop_small=not keep)
+        assert np.array_equal(rp, orp), "row lengths differ"
+        assert np.array_equal(c, oc), "columns differ"
+        assert np.array_equal(v.view(np.uint64), ov.view(np.uint64)), "values not bit-exact"
+        nu, nf = op.nnz()
+        assert nu == orp[-1]
+        if not keep:
+            assert nu == g.meta["nnz_upper_csb"]
+        else:
+            assert nu == g.meta["hh_nnz"]
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_rows_match_reference_hh(gpu, golden, name):
+    g = golden(name)
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml), gpu.OP_KEEP_SMALL)
+    rp, c, v = op.upper_rows()
+    a = g.arrays
+    assert np.array_equal(rp, a["hh_rowptr"]) and np.array_equal(c, a["hh_cols"])
+    assert np.max(np.abs(v - a["hh_vals"])) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["sd4p4n_s0", "fp3p2n_m1_s3"])
+def test_sampled_rows_match_reference_hh(gpu, golden, name):
+    g = golden(name)
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml), gpu.OP_KEEP_SMALL)
+    a = g.arrays
+    for i, r in enumerate(a["sample_rows"]):
+        rp, c, v = op.upper_rows(int(r), int(r) + 1)
+        s0, s1 = a["sample_rowptr"][i], a["sample_rowptr"][i + 1]
+        assert np.array_equal(c, a["sample_cols"][s0:s1])
+        assert np.max(np.abs(v - a["sample_vals"][s0:s1])) < 1e-12
+
+
+def test_hh_file_roundtrip(gpu, golden, tmp_path):
+    g = golden("sd2p2n_s11")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml), gpu.OP_KEEP_SMALL)
+    p = str(tmp_path / "x.HH")
+    op.write_hh(p)
+    rp, c, v = oracle.read_hh(p)
+    orp, oc, ov = oracle.System(g.xml).h_upper(drop_small=False)
+    assert np.array_equal(rp, orp) and np.array_equal(c, oc) and np.array_equal(v, ov)
+    op2 = gpu.Operator.read_hh(p)
+    assert np.array_equal(op2.diagonal(), op.diagonal())
+
+
+def test_diagonal_and_symmetry(gpu, golden):
+    g = golden("fp2p2n_s5")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    rp, c, v = op.upper_rows()
+    d = op.diagonal()
+    n = op.n
+    assert np.array_equal(d, v[rp[:-1]])  # diagonal first in every row
+    assert np.all(c[rp[:-1]] == np.arange(n))
+    H = oracle.dense_from_upper(n, rp, c, v)
+    X = np.random.default_rng(3).standard_normal((n, 2))
+    Y = op.apply(X)
+    assert np.allclose(Y, H @ X, rtol=0, atol=1e-11)
+
+
+def test_virtual_diagonal_equals_stored(gpu, golden):
+    g = golden("sd4p4n_s0")
+    s = gpu.System.from_xml(g.xml)
+    a = gpu.Operator.build(s)
+    b = gpu.Operator.build(s, gpu.OP_VIRTUAL)
+    assert np.array_equal(a.diagonal(), b.diagonal())

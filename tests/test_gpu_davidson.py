@@ -1,0 +1,85 @@
+"""Block Davidson on the GPU against dense eigensolves of the oracle H and against the
+reference Davidson's eigenvalues (where the reference's own residual is small)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import FULL_CASES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_lowest_pairs_vs_dense(gpu, golden, name):
+    g = golden(name)
+    s = gpu.System.from_xml(g.xml)
+    op = gpu.Operator.build(s)
+    o = oracle.System(g.xml)
+    rp, c, v = o.h_upper()
+    H = oracle.dense_from_upper(op.n, rp, c, v)
+    w = np.linalg.eigvalsh(H)
+    k = min(8, op.n)
+    ev, de, rs, vec, st = gpu.davidson(op, k)
+    assert np.max(np.abs(ev - w[:k])) < 1e-8
+    assert np.max(rs) < 1e-6
+    R = H @ vec - vec * ev
+    assert np.max(np.linalg.norm(R, axis=0)) < 1e-6
+    assert np.allclose(vec.T @ vec, np.eye(k), atol=1e-8)
+
+
+@pytest.mark.parametrize("name", ["sd4p4n_s0", "fp3p2n_m1_s3"])
+def test_reference_eigenvalues(gpu, golden, name):
+    g = golden(name)
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    k = g.meta["davidson_block"]
+    ev, de, rs, vec, st = gpu.davidson(op, k, want_vectors=False)
+    ref = np.array(g.meta["ref_evals"])
+    rres = np.array(g.meta["ref_resid"])
+    ok = rres <= 1e-6
+    assert ok.sum() >= k // 2
+    assert np.max(np.abs(ev[ok] - ref[ok])) < 1e-7
+    assert np.max(rs) < 1e-6
+
+
+def test_mixed_precision_within_1kev(gpu, golden):
+    g = golden("sd4p4n_s0")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    e64, _, r64, _, st64 = gpu.davidson(op, 10, want_vectors=False, critr=1e-6)
+    e32, _, r32, _, st32 = gpu.davidson(op, 10, want_vectors=False, critr=1e-6, precision=gpu.PREC_FP32)
+    assert np.max(np.abs(e32 - e64)) <= 1e-3
+    assert np.max(r32) <= 1e-6
+    assert st32["matvecs_fp64"] < st32["matvecs"]
+
+
+def test_block_smaller_than_pairs(gpu, golden):
+    g = golden("fp2p2n_s5")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    o = oracle.System(g.xml)
+    rp, c, v = o.h_upper()
+    w = np.linalg.eigvalsh(oracle.dense_from_upper(op.n, rp, c, v))
+    ev, de, rs, vec, st = gpu.davidson(op, 10, mblock=4, want_vectors=False)
+    assert np.max(np.abs(ev - w[:10])) < 1e-8
+
+
+def test_virtual_operator_solve(gpu, golden):
+    g = golden("sd2p2n_s11")
+    s = gpu.System.from_xml(g.xml)
+    a = gpu.Operator.build(s)
+    b = gpu.Operator.build(s, gpu.OP_VIRTUAL)
+    ea = gpu.davidson(a, 6, want_vectors=False)[0]
+    eb = gpu.davidson(b, 6, want_vectors=False)[0]
+    assert np.max(np.abs(ea - eb)) < 1e-9
+
+
+def test_ee_roundtrip(gpu, golden, tmp_path):
+    g = golden("sd2p2n_s11")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    ev, de, rs, vec, st = gpu.davidson(op, 4)
+    p = str(tmp_path / "x.EE")
+    gpu.write_ee(p, ev, de, rs, vec)
+    E, dE, R, V = oracle.read_ee(p)
+    assert np.array_equal(E, ev) and np.array_equal(R, rs)
+    assert np.array_equal(np.stack(V, 1), vec)
+    # warm start from the written vectors converges at once to the same values
+    ev2 = gpu.davidson(op, 4, init=vec, want_vectors=False)[0]
+    assert np.max(np.abs(ev2 - ev)) < 1e-9

@@ -1,0 +1,81 @@
+"""H.X on the GPU: ordered FP64 is bit-identical to the reference CSB product order;
+fast FP64 / FP32 / virtual within stated tolerances."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import FULL_CASES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", FULL_CASES + ["sd4p4n_s0"])
+def test_ordered_fp64_bit_exact_vs_csb_product(gpu, golden, name):
+    g = golden(name)
+    s = gpu.System.from_xml(g.xml)
+    o = oracle.System(g.xml)
+    op = gpu.Operator.build(s)
+    rp, c, v = o.h_upper(drop_small=True)
+    n = op.n
+    X = np.sin(0.37 * np.arange(n)[:, None] + 1.3 * np.arange(3)[None, :])
+    Yo = oracle.csb_times(n, rp, c, v, X)
+    Yg = op.apply(X, gpu.PREC_FP64_ORDERED)
+    assert np.array_equal(Yg.view(np.uint64), Yo.view(np.uint64))
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_reference_product_on_reference_values(gpu, golden, name):
+    """The reference's own TimesMatrix output (oracle/_ref harness) on its own H: the GPU
+    operator uploaded from the reference .HH rows reproduces it bit for bit."""
+    g = golden(name)
+    a = g.arrays
+    n = g.meta["n"]
+    op = gpu.Operator.from_upper(n, a["hh_rowptr"], a["hh_cols"], a["hh_vals"])  # drops |h|<1e-14 like CSB
+    Y = op.apply(a["spmm_x"], gpu.PREC_FP64_ORDERED)
+    assert np.array_equal(Y.view(np.uint64), a["spmm_y"].view(np.uint64))
+
+
+@pytest.mark.parametrize("m", [1, 3, 8, 10, 16])
+def test_fast_fp64_and_fp32(gpu, golden, m):
+    g = golden("sd4p4n_s0")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    o = oracle.System(g.xml)
+    rp, c, v = o.h_upper(drop_small=True)
+    n = op.n
+    X = np.random.default_rng(m).standard_normal((n, m))
+    ref = oracle.csb_times(n, rp, c, v, X)
+    scale = np.abs(ref).max()
+    y64 = op.apply(X, gpu.PREC_FP64)
+    assert np.max(np.abs(y64 - ref)) < 1e-12 * scale
+    y32 = op.apply(X, gpu.PREC_FP32)
+    assert np.max(np.abs(y32 - ref)) < 2e-5 * scale
+
+
+@pytest.mark.parametrize("name", ["sd4p4n_s0", "fp2p2n_s5"])
+def test_virtual_equals_stored(gpu, golden, name):
+    g = golden(name)
+    s = gpu.System.from_xml(g.xml)
+    a = gpu.Operator.build(s)
+    b = gpu.Operator.build(s, gpu.OP_VIRTUAL)
+    n = a.n
+    X = np.random.default_rng(7).standard_normal((n, 5))
+    ya = a.apply(X, gpu.PREC_FP64)
+    yb = b.apply(X, gpu.PREC_FP64)
+    assert np.max(np.abs(ya - yb)) < 1e-12 * np.abs(ya).max()
+    yb32 = b.apply(X, gpu.PREC_FP32)
+    assert np.max(np.abs(yb32 - ya)) < 2e-5 * np.abs(ya).max()
+
+
+def test_apply_device_torch(gpu, golden):
+    import torch
+    g = golden("sd4p4n_s0")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml))
+    n = op.n
+    X = torch.randn(n, 16, dtype=torch.float64, device="cuda")
+    Y = torch.empty_like(X)
+    op.apply_device(X, Y, gpu.PREC_FP64)
+    torch.cuda.synchronize()
+    ref = op.apply(X.cpu().numpy(), gpu.PREC_FP64)
+    assert np.max(np.abs(Y.cpu().numpy() - ref)) < 1e-12 * np.abs(ref).max()
