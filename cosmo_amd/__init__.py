@@ -33,7 +33,7 @@ class DavParams(ctypes.Structure):
     _fields_ = [("nume", ctypes.c_int), ("mblock", ctypes.c_int), ("lim", ctypes.c_int), ("maxiter", ctypes.c_int),
                 ("crite", ctypes.c_double), ("critc", ctypes.c_double), ("critr", ctypes.c_double),
                 ("ortho", ctypes.c_double), ("precision", ctypes.c_int), ("mixed_switch", ctypes.c_double),
-                ("ninit", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("ninit", ctypes.c_int), ("verbose", ctypes.c_int), ("strict", ctypes.c_int)]
 
 
 class DavStats(ctypes.Structure):

@@ -150,13 +150,14 @@ __global__ void k_precond(int64_t n, int q, double *__restrict__ R, int64_t ldr,
 }
 
 // column-major B[:, c0 .. c0+m) -> interleaved X (n x ldx), as double or float
+// (columns m .. ldx-1 of X are zero padding)
 template <typename T>
 __global__ void k_cols_to_rows(int64_t n, int m, const double *__restrict__ B, int64_t ldb, T *__restrict__ X, int ldx) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * m) return;
-  int64_t i = t / m;
-  int j = (int)(t % m);
-  X[i * ldx + j] = (T)B[(int64_t)j * ldb + i];
+  if (t >= n * ldx) return;
+  int64_t i = t / ldx;
+  int j = (int)(t % ldx);
+  X[i * ldx + j] = j < m ? (T)B[(int64_t)j * ldb + i] : (T)0;
 }
 template <typename T>
 __global__ void k_rows_to_cols(int64_t n, int m, const T *__restrict__ Y, int ldy, double *__restrict__ B, int64_t ldb) {
@@ -202,10 +203,12 @@ static int dav_alloc(Dav &D) {
   HIPCHK(hipMalloc(&D.AB, sizeof(double) * n * D.lim));
   HIPCHK(hipMalloc(&D.R, sizeof(double) * n * std::max(D.nume, D.mblock)));
   HIPCHK(hipMalloc(&D.T, sizeof(double) * n * std::max(D.lim, 16)));
-  HIPCHK(hipMalloc(&D.Xd, sizeof(double) * n * D.mblock));
-  HIPCHK(hipMalloc(&D.Yd, sizeof(double) * n * D.mblock));
-  HIPCHK(hipMalloc(&D.Xf, sizeof(float) * n * D.mblock));
-  HIPCHK(hipMalloc(&D.Yf, sizeof(float) * n * D.mblock));
+  int ldmax = 4;
+  while (ldmax < D.mblock) ldmax <<= 1;
+  HIPCHK(hipMalloc(&D.Xd, sizeof(double) * n * ldmax));
+  HIPCHK(hipMalloc(&D.Yd, sizeof(double) * n * ldmax));
+  HIPCHK(hipMalloc(&D.Xf, sizeof(float) * n * ldmax));
+  HIPCHK(hipMalloc(&D.Yf, sizeof(float) * n * ldmax));
   D.G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + GRAM_ROWS - 1) / GRAM_ROWS));
   HIPCHK(hipMalloc(&D.partial, sizeof(double) * (size_t)D.G * 1024));
   HIPCHK(hipMalloc(&D.small, sizeof(double) * 8192));
@@ -294,19 +297,22 @@ static int apply_block(Dav &D, int c0, int m, int prec) {
   auto t0 = std::chrono::steady_clock::now();
   for (int j0 = 0; j0 < m; j0 += D.mblock) {
     int mm = std::min(D.mblock, m - j0);
-    int64_t tot = D.n * mm;
-    dim3 g((unsigned)((tot + 255) / 256)), b(256);
+    // leading dimension padded to a power of two of at least 16 bytes (vectorised H.X)
+    int ld = (prec == COSMO_PREC_FP32) ? 4 : 2;
+    while (ld < mm) ld <<= 1;
+    int64_t tot = D.n * mm, totp = D.n * ld;
+    dim3 g((unsigned)((tot + 255) / 256)), gp((unsigned)((totp + 255) / 256)), b(256);
     int rc;
     if (prec == COSMO_PREC_FP32) {
-      hipLaunchKernelGGL(k_cols_to_rows<float>, g, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xf, mm);
-      rc = op_apply_dev(*D.op, D.Xf, D.Yf, mm, mm, mm, COSMO_PREC_FP32, D.st);
+      hipLaunchKernelGGL(k_cols_to_rows<float>, gp, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xf, ld);
+      rc = op_apply_dev(*D.op, D.Xf, D.Yf, mm, ld, ld, COSMO_PREC_FP32, D.st);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_rows_to_cols<float>, g, b, 0, D.st, D.n, mm, D.Yf, mm, D.col(D.AB, c0 + j0), D.n);
+      hipLaunchKernelGGL(k_rows_to_cols<float>, g, b, 0, D.st, D.n, mm, D.Yf, ld, D.col(D.AB, c0 + j0), D.n);
     } else {
-      hipLaunchKernelGGL(k_cols_to_rows<double>, g, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xd, mm);
-      rc = op_apply_dev(*D.op, D.Xd, D.Yd, mm, mm, mm, COSMO_PREC_FP64, D.st);
+      hipLaunchKernelGGL(k_cols_to_rows<double>, gp, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xd, ld);
+      rc = op_apply_dev(*D.op, D.Xd, D.Yd, mm, ld, ld, COSMO_PREC_FP64, D.st);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_rows_to_cols<double>, g, b, 0, D.st, D.n, mm, D.Yd, mm, D.col(D.AB, c0 + j0), D.n);
+      hipLaunchKernelGGL(k_rows_to_cols<double>, g, b, 0, D.st, D.n, mm, D.Yd, ld, D.col(D.AB, c0 + j0), D.n);
       D.matvecs64 += mm;
     }
     HIPCHK(hipGetLastError());
@@ -512,6 +518,7 @@ int cosmo_dav_defaults(cosmo_dav_params *p, int nume) {
   p->mixed_switch = 0.0;
   p->ninit = 0;
   p->verbose = 0;
+  p->strict = 1;
   return COSMO_OK;
 }
 
@@ -621,7 +628,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
           if (icv[i] == 0) {
             double tmax = std::fabs(svec[i][kpass - 1]);
             for (int l = 1; l <= nncv - 1; ++l) tmax = std::max(tmax, std::fabs(svec[i][kpass - l - 1]));
-            if (tmax < p.critc) icv[i] = 1;
+            if (tmax < p.critc && !p.strict) icv[i] = 1;
             else {
               ind.push_back(i + 1);
               rowlast.push_back(tmax);

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "device.h"
@@ -398,6 +399,98 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t *__restri
   if (slot == 0 && act) Y[row * ldy + v] = (XT)acc;
 }
 
+// Stored H.X, vectorised form: X, Y row-major with ld = LPE * (16 / sizeof(T)) so one
+// X row is LPE 16-byte pieces.  A wavefront owns a row; per 64-entry chunk every lane
+// loads one (col, value) pair (coalesced, prefetched one chunk ahead), then LPE steps
+// broadcast the entries to lane groups of LPE lanes, each lane gathering one 16-byte
+// piece of its entry's X row.  Partial sums are reduced across groups at the row end.
+// Workgroups are remapped so that each XCD (dispatched round-robin) sweeps a contiguous
+// range of rows: neighbouring rows touch neighbouring X rows, which then share an L2.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int N = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int N = 2;
+};
+
+__device__ __forceinline__ int64_t xcd_row_block(int remap) {
+  const int64_t G = gridDim.x, i = blockIdx.x;
+  if (!remap) return i;
+  const int64_t q = G / 8, r = G % 8, x = i % 8;
+  return x * q + (x < r ? x : r) + i / 8;
+}
+
+template <typename T, int LPE>
+__global__ __launch_bounds__(256) void k_spmm_vec(int64_t n, const int64_t *__restrict__ rowptr,
+                                                  const uint32_t *__restrict__ cols, const T *__restrict__ vals,
+                                                  const T *__restrict__ X, T *__restrict__ Y, int m, int ld,
+                                                  int remap) {
+  using V = typename Vec16<T>::type;
+  constexpr int VPL = Vec16<T>::N;
+  constexpr int EPS = 64 / LPE;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = xcd_row_block(remap) * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int grp = lane / LPE, sub = lane % LPE;
+  const int64_t s = rowptr[row], e = rowptr[row + 1];
+  T acc[VPL];
+#pragma unroll
+  for (int q = 0; q < VPL; ++q) acc[q] = 0;
+  uint32_t c = 0;
+  T h = 0;
+  if (s + lane < e) {
+    c = cols[s + lane];
+    h = vals[s + lane];
+  }
+  for (int64_t base = s; base < e; base += 64) {
+    const int64_t nidx = base + 64 + lane;
+    uint32_t cn = 0;
+    T hn = 0;
+    if (nidx < e) {
+      cn = cols[nidx];
+      hn = vals[nidx];
+    }
+    const int cnt = (int)min<int64_t>(64, e - base);
+#pragma unroll
+    for (int st = 0; st < LPE; ++st) {
+      const int j = st * EPS + grp;
+      const uint32_t cj = __shfl(c, j);
+      const T hj = __shfl(h, j);
+      if (j < cnt) {
+        const V xv = *reinterpret_cast<const V *>(X + (int64_t)cj * ld + sub * VPL);
+        const T *xs = reinterpret_cast<const T *>(&xv);
+#pragma unroll
+        for (int q = 0; q < VPL; ++q) acc[q] += hj * xs[q];
+      }
+    }
+    c = cn;
+    h = hn;
+  }
+#pragma unroll
+  for (int q = 0; q < VPL; ++q)
+    for (int off = LPE; off < 64; off <<= 1) acc[q] += __shfl_xor(acc[q], off);
+  if (grp == 0) {
+    T *y = Y + row * ld + sub * VPL;
+    if ((sub + 1) * VPL <= m) {
+      V o;
+      T *os = reinterpret_cast<T *>(&o);
+#pragma unroll
+      for (int q = 0; q < VPL; ++q) os[q] = acc[q];
+      *reinterpret_cast<V *>(y) = o;
+    } else {
+#pragma unroll
+      for (int q = 0; q < VPL; ++q)
+        if (sub * VPL + q < m) y[q] = acc[q];
+    }
+  }
+}
+
 // FP64 product summed per output element over the full row in ascending column order,
 // one multiply then one add per entry (no contraction): the summation order of
 // SparseMatrixCSB::TimesMatrix for a symmetric matrix (include/tav/csbmatrix.cxx:623-663,
@@ -689,9 +782,38 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
 
 // ------------------------------------------------------------------------------------
 // products
+int g_spmm_remap = 1;  // XCD-contiguous row ranges (COSMO_SPMM_REMAP=0 turns it off)
+
+// vectorised kernel when X and Y share a power-of-two leading dimension of >= 16 bytes
+template <typename T>
+static bool launch_spmm_vec(const CsrDev &M, const T *vals, const T *X, T *Y, int m, int ld, hipStream_t st) {
+  constexpr int VPL = 16 / sizeof(T);
+  if (ld < VPL || (ld & (ld - 1)) || ld > 64 || m > ld) return false;
+  if ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y)) & 15) return false;
+  const int lpe = ld / VPL;
+  dim3 g((unsigned)((M.n + 3) / 4)), b(256);
+  const int rm = g_spmm_remap;
+  switch (lpe) {
+    case 1: hipLaunchKernelGGL((k_spmm_vec<T, 1>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    case 2: hipLaunchKernelGGL((k_spmm_vec<T, 2>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    case 4: hipLaunchKernelGGL((k_spmm_vec<T, 4>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    case 8: hipLaunchKernelGGL((k_spmm_vec<T, 8>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    case 16: hipLaunchKernelGGL((k_spmm_vec<T, 16>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    case 32: hipLaunchKernelGGL((k_spmm_vec<T, 32>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
+    default: return false;
+  }
+  return true;
+}
+
 template <typename VT, typename XT, typename AT>
 static int launch_spmm_t(const CsrDev &M, const VT *vals, const XT *X, XT *Y, int m, int ldx, int ldy,
                          hipStream_t st) {
+  if constexpr (std::is_same<VT, XT>::value) {
+    if (ldx == ldy && launch_spmm_vec<XT>(M, vals, X, Y, m, ldx, st)) {
+      HIPCHK(hipGetLastError());
+      return COSMO_OK;
+    }
+  }
   dim3 g((unsigned)((M.n + 3) / 4)), b(256);
   if (m <= 1) hipLaunchKernelGGL((k_spmm<VT, XT, AT, 1>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ldx, ldy);
   else if (m <= 2) hipLaunchKernelGGL((k_spmm<VT, XT, AT, 2>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ldx, ldy);
@@ -720,6 +842,11 @@ static int launch_virtual_t(const DevTables &T, int64_t n, const XT *X, XT *Y, i
 // Y = H X on device, X/Y row-major with leading dimensions; m <= 64 (virtual: <= 16)
 int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int prec, hipStream_t st) {
   if (m <= 0) return COSMO_OK;
+  static bool env_done = false;
+  if (!env_done) {
+    if (const char *e = getenv("COSMO_SPMM_REMAP")) g_spmm_remap = atoi(e);
+    env_done = true;
+  }
   if (op.is_virtual) {
     for (int v0 = 0; v0 < m; v0 += 16) {
       int mm = std::min(16, m - v0);

@@ -157,6 +157,11 @@ typedef struct {
   int ninit;           /* number of initial vectors given (0: unit vectors at the lowest
                           diagonal elements, fsdavidson.cxx:540-611)                    */
   int verbose;
+  int strict;          /* 1 (default): a pair counts as converged only when its residual
+                          is below critr.  0: the reference's test, which also accepts a
+                          pair whose last Ritz coefficients fall under critc and keeps it
+                          converged for good (fsdavidson.cxx:243-256) -- that test can
+                          end with unconverged pairs                                      */
 } cosmo_dav_params;
 
 typedef struct {
