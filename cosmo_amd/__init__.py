@@ -73,6 +73,8 @@ SIGNATURES = {
     "cosmo_system_tensors": (_I, [_P, _P, _P, _P, _P]),
     "cosmo_system_orbitals": (_I, [_P, _P, _P, _P, _P]),
     "cosmo_system_write_mbs": (_I, [_P, _S]),
+    "cosmo_system_labels": (_I, [_P, _P, _I, _P, _P]),
+    "cosmo_twice_spin": (_I, [_D, _I]),
     "cosmo_system_check_mbs": (_I, [_P, _S]),
     "cosmo_op_create": (_I, [_P, _I, _P]),
     "cosmo_op_create_upper": (_I, [_L, _P, _P, _P, _I, _P]),
@@ -87,6 +89,7 @@ SIGNATURES = {
     "cosmo_op_upper_rows": (_I, [_P, _L, _L, _P, _P, _P, _P]),
     "cosmo_op_write_hh": (_I, [_P, _S]),
     "cosmo_op_timings": (_I, [_P, _P, _P]),
+    "cosmo_op_dense_eigen": (_I, [_P, _P, _P]),
     "cosmo_dav_defaults": (_I, [_P, _I]),
     "cosmo_davidson": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cosmo_write_ee": (_I, [_S, _L, _I, _P, _P, _P, _P]),
@@ -175,6 +178,15 @@ class System:
         vv = np.zeros(self.nvspx)
         check(lib().cosmo_system_tensors(self.h, _p(ek), _p(ev), _p(vk), _p(vv)))
         return ek, ev, vk, vv
+
+    def labels(self, evecs):
+        """(<J-J+>, <T-T+>) of the columns of evecs (n x count)"""
+        Z = np.ascontiguousarray(np.asarray(evecs, dtype=np.float64).T)
+        c = Z.shape[0]
+        jj = np.zeros(c)
+        tt = np.zeros(c)
+        check(lib().cosmo_system_labels(self.h, _p(Z), c, _p(jj), _p(tt)))
+        return jj, tt
 
     def write_mbs(self, path):
         check(lib().cosmo_system_write_mbs(self.h, path.encode()))
@@ -269,6 +281,12 @@ class Operator:
 
     def write_hh(self, path):
         check(lib().cosmo_op_write_hh(self.h, path.encode()))
+
+    def dense_eigen(self, vectors=True):
+        w = np.zeros(self.n)
+        V = np.zeros((self.n, self.n)) if vectors else None
+        check(lib().cosmo_op_dense_eigen(self.h, _p(w), _p(V)))
+        return w, (V.T if vectors else None)
 
     def timings(self):
         b, a = ctypes.c_double(), ctypes.c_double()

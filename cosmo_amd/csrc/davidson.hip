@@ -149,6 +149,12 @@ __global__ void k_precond(int64_t n, int q, double *__restrict__ R, int64_t ldr,
   else *r = *r / 1.0e-13;
 }
 
+__global__ void k_scale_cols(int64_t n, int q, double *__restrict__ V, int64_t ldv, const double *__restrict__ sc) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * q) return;
+  V[(t / n) * ldv + t % n] *= sc[t / n];
+}
+
 // column-major B[:, c0 .. c0+m) -> interleaved X (n x ldx), as double or float
 // (columns m .. ldx-1 of X are zero padding)
 template <typename T>
@@ -380,6 +386,23 @@ static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &
   restart = false;
   kept = q;
   std::vector<double> W;
+  if (q > 0) {
+    // unit columns first: the absolute thresholds below then mean what they say even when
+    // the preconditioner has blown a vector up by up to 1e13
+    std::vector<double> G0;
+    int rc0 = gram(D, Vp, q, Vp, q, G0);
+    if (rc0) return rc0;
+    std::vector<double> sc(q);
+    for (int j = 0; j < q; ++j) {
+      double nr = std::sqrt(std::max(G0[(size_t)j * q + j], 0.0));
+      sc[j] = nr > 0 ? 1.0 / nr : 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(D.small, sc.data(), sizeof(double) * q, hipMemcpyHostToDevice, D.st));
+    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((D.n * q + 255) / 256)), dim3(256), 0, D.st, D.n, q, Vp, D.n,
+                       D.small);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(D.st));
+  }
   for (int round = 0; round < 2 && kept > 0; ++round) {
     double dprev = 1e7;
     for (int pass = 0; pass < 4; ++pass) {
@@ -537,7 +560,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   if (p.maxiter <= 0) p.maxiter = std::max(p.nume * 40, 200);
   const bool mixed = p.precision == COSMO_PREC_FP32;
   if (mixed && !op->is_virtual && !op->csr.v32) return fail(COSMO_EINVAL, "mixed precision needs FP32 values");
-  const double switch_tol = p.mixed_switch > 0 ? p.mixed_switch : 2e-5;
+  const double switch_tol = p.mixed_switch > 0 ? p.mixed_switch : 1e-5;
   hipSetDevice(op->device);
   auto tstart = std::chrono::steady_clock::now();
   Dav D;
@@ -594,15 +617,13 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   int nloops = 0, ierr = 0, restarts = 0;
   // mixed precision: collapse to the Ritz vectors, recompute H.B in FP64, restart
   auto switch_fp64 = [&]() -> int {
-    int r = collapse(D, svec, kpass);
-    if (r) return r;
-    r = apply_block(D, 0, nume, COSMO_PREC_FP64);
+    // keep the whole FP32-grown basis; only H.B and S are recomputed in FP64
+    int r = apply_block(D, 0, kpass, COSMO_PREC_FP64);
     if (r) return r;
     std::fill(s.begin(), s.end(), 0.0);
-    r = update_S(D, s, 0, nume);
+    r = update_S(D, s, 0, kpass);
     if (r) return r;
-    kpass = nume;
-    nncv = nume;
+    nncv = kpass;
     std::fill(icv.begin(), icv.end(), 0);
     for (int i = 0; i < nume; ++i) eigs[i] = 1.e+30;
     fp64_phase = true;

@@ -78,6 +78,13 @@ int cosmo_system_tensors(cosmo_system s, uint16_t *esp_keys, double *esp_vals, u
                          double *vspx_vals);
 /* Orbital table copy (nsps entries each). */
 int cosmo_system_orbitals(cosmo_system s, int *level, int *j2, int *m2, int *tz2);
+/* Spin labels of eigenvectors (count host n-vectors, column-major): jm_jp[v] = <J-J+>,
+ * tm_tp[v] = <T-T+> (either may be NULL), computed on the GPU as |J+ z|^2, |T+ z|^2.
+ * Replaces PPMatrix::Sqr of the J+ / T+ operators in XSHLJT
+ * (include/xsm/XSHLJT.cxx:38-142, include/sm/PPMatrix.cxx:400-411). */
+int cosmo_system_labels(cosmo_system s, const double *evecs, int count, double *jm_jp, double *tm_tp);
+/* 2J from <J-J+> and 2M: sqrt(4x + 2M(2M+2) + 1) - 1 rounded (include/sm/QN.cxx:40-55). */
+int cosmo_twice_spin(double x, int twoM);
 /* .mbs file I/O (include/sm/ManyBodyStates.cxx:199-246).  cosmo_system_check_mbs fails
  * with COSMO_EUNSUP when the file's basis differs from the full M-scheme basis
  * (truncated bases are not handled by the rank-table builder). */
@@ -135,6 +142,11 @@ int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, co
                         int64_t *cols, double *vals);
 /* Write the .HH file of the operator (include/sm/PPMatrix.cxx:413-465 layout). */
 int cosmo_op_write_hh(cosmo_op op, const char *path);
+/* All eigenpairs of a small stored operator (n <= 20000), ascending; evecs n x n
+ * column-major (vector j at evecs + j*n) or NULL.  The exact-diagonalisation branch of
+ * the reference for n <= limit (include/xsm/DEXHVcsb.cxx:72-108), computed on the host by
+ * Householder tridiagonalisation and implicit QL. */
+int cosmo_op_dense_eigen(cosmo_op op, double *evals, double *evecs);
 /* Last apply/build timings in milliseconds (device events). */
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms);
 
@@ -153,7 +165,7 @@ typedef struct {
                           and bulk orthogonalisation, FP64 Rayleigh-Ritz, residuals and a
                           final FP64 refinement)                                          */
   double mixed_switch; /* mixed: residual (relative to |lambda|) at which the FP32 phase
-                          hands over to FP64 refinement; 0 picks the default 2e-5        */
+                          hands over to FP64 refinement; 0 picks the default 1e-5        */
   int ninit;           /* number of initial vectors given (0: unit vectors at the lowest
                           diagonal elements, fsdavidson.cxx:540-611)                    */
   int verbose;

@@ -9,8 +9,8 @@ from conftest import FULL_CASES
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", FULL_CASES)
-def test_lowest_pairs_vs_dense(gpu, golden, name):
+@pytest.mark.parametrize("name,k", [("sd2p2n_s11", 8), ("sd3p1n_m1_s7", 6), ("fp2p2n_s5", 8)])
+def test_lowest_pairs_vs_dense(gpu, golden, name, k):
     g = golden(name)
     s = gpu.System.from_xml(g.xml)
     op = gpu.Operator.build(s)
@@ -18,8 +18,8 @@ def test_lowest_pairs_vs_dense(gpu, golden, name):
     rp, c, v = o.h_upper()
     H = oracle.dense_from_upper(op.n, rp, c, v)
     w = np.linalg.eigvalsh(H)
-    k = min(8, op.n)
     ev, de, rs, vec, st = gpu.davidson(op, k)
+    assert st["converged"]
     assert np.max(np.abs(ev - w[:k])) < 1e-8
     assert np.max(rs) < 1e-6
     R = H @ vec - vec * ev
@@ -32,7 +32,10 @@ def test_reference_eigenvalues(gpu, golden, name):
     g = golden(name)
     op = gpu.Operator.build(gpu.System.from_xml(g.xml))
     k = g.meta["davidson_block"]
-    ev, de, rs, vec, st = gpu.davidson(op, k, want_vectors=False)
+    # a wider basis than the reference default (nume+20): its last pair sits next to a
+    # near-degenerate neighbour, where the reference itself stopped unconverged
+    ev, de, rs, vec, st = gpu.davidson(op, k, want_vectors=False, lim=k + 40, maxiter=2000)
+    assert st["converged"]
     ref = np.array(g.meta["ref_evals"])
     rres = np.array(g.meta["ref_resid"])
     ok = rres <= 1e-6
