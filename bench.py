@@ -139,7 +139,7 @@ def solve_section(ca, op, n):
     res = {}
     for label, prec in (("fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
         t0 = time.perf_counter()
-        ev, de, rs, _, st = ca.davidson(op, 10, mblock=16, lim=36, want_vectors=False, critr=1e-6, precision=prec)
+        ev, de, rs, _, st = ca.davidson(op, 10, mblock=16, lim=60, want_vectors=False, critr=1e-6, precision=prec)
         dt = time.perf_counter() - t0
         res[label] = dict(seconds=round(dt, 3), evals=[round(float(x), 8) for x in ev],
                           max_resid=float(np.max(rs)), iterations=st["iterations"], matvecs=st["matvecs"],
