@@ -83,6 +83,7 @@ SIGNATURES = {
     "cosmo_op_size": (_L, [_P]),
     "cosmo_op_nnz": (_I, [_P, _P, _P]),
     "cosmo_op_bytes": (_L, [_P]),
+    "cosmo_op_footprint": (_I, [_P, _I, _P]),
     "cosmo_op_diagonal": (_I, [_P, _P]),
     "cosmo_op_apply": (_I, [_P, _P, _P, _I, _I]),
     "cosmo_op_apply_device": (_I, [_P, _P, _P, _I, _I, _I, _P]),
@@ -337,3 +338,98 @@ def write_ee(path, evals, dE, resid, evecs):
     n = evecs.shape[1]
     check(lib().cosmo_write_ee(path.encode(), n, len(evals), _p(np.asarray(evals, float)), _p(np.asarray(dE, float)),
                                _p(np.asarray(resid, float)), _p(evecs)))
+
+
+def footprint(system: System, flags=OP_STORED):
+    """Exact device bytes a stored operator of `system` would take."""
+    b = ctypes.c_int64()
+    check(lib().cosmo_op_footprint(system.h, flags, ctypes.byref(b)))
+    return b.value
+
+
+class Batch:
+    """Independent solves run together on the device (one stream per worker)."""
+
+    def __init__(self, systems, flags=OP_STORED):
+        self.systems = list(systems)
+        arr = (ctypes.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        self.h = ctypes.c_void_p()
+        check(lib().cosmo_batch_create(len(self.systems), arr, flags, ctypes.byref(self.h)))
+
+    def solve(self, nume, **kw):
+        """nume: int or per-system list.  Returns (list of eigenvalue arrays, list of
+        residual arrays, stats dict)."""
+        nl = [nume] * len(self.systems) if np.isscalar(nume) else list(nume)
+        p = dav_params(max(nl), **kw)
+        cnt = np.array(nl, dtype=np.int32)
+        tot = int(cnt.sum())
+        ev = np.zeros(tot)
+        rs = np.zeros(tot)
+        st = DavStats()
+        rc = lib().cosmo_batch_solve(self.h, ctypes.byref(p), _p(cnt), _p(ev), _p(rs), ctypes.byref(st))
+        if rc != ECONV:
+            check(rc)
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        return ([ev[off[i]:off[i + 1]] for i in range(len(nl))], [rs[off[i]:off[i + 1]] for i in range(len(nl))],
+                dict(st.as_dict(), converged=(rc == OK)))
+
+    def close(self):
+        if self.h:
+            lib().cosmo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop: admission by HBM footprint, up to max_active solves overlapping."""
+
+    def __init__(self, hbm_budget, max_active=8, precision=PREC_FP32):
+        p = ServerParams(int(hbm_budget), int(max_active), int(precision))
+        self.h = ctypes.c_void_p()
+        check(lib().cosmo_server_create(ctypes.byref(p), ctypes.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, system: System, nume, tol=1e-6):
+        i = ctypes.c_int64()
+        check(lib().cosmo_server_submit(self.h, system.h, int(nume), float(tol), ctypes.byref(i)))
+        self._keep[i.value] = system  # the system must outlive its request
+        return i.value
+
+    @staticmethod
+    def _conv(r):
+        k = r.nume
+        return dict(id=r.id, n=r.n, nume=k, status=r.status, t_submit=r.t_submit, t_admit=r.t_admit, t_done=r.t_done,
+                    evals=np.array(r.evals[:k]), resid=np.array(r.resid[:k]))
+
+    def poll(self, cap=256):
+        buf = (Result * cap)()
+        k = check(lib().cosmo_server_poll(self.h, buf, cap))
+        out = [self._conv(buf[i]) for i in range(k)]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
+
+    def drain(self, cap=4096):
+        buf = (Result * cap)()
+        w = ctypes.c_int()
+        check(lib().cosmo_server_drain(self.h, buf, cap, ctypes.byref(w)))
+        out = [self._conv(buf[i]) for i in range(w.value)]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().cosmo_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -51,7 +51,10 @@ int main(int argc, char **argv) {
   if (mbs && mbs->get("name")) {
     std::string p = mbs->get("name");
     if (p[0] != '/' && !std::ifstream(p)) p = dir + "/" + p;
-    if (cosmo_system_check_mbs(s, p.c_str())) return die("many-body states");
+    if (!std::ifstream(p)) {  // named but missing: generate it again
+      if (cosmo_system_write_mbs(s, p.c_str())) return die("write .mbs");
+    } else if (cosmo_system_check_mbs(s, p.c_str()))
+      return die("many-body states");
   }
   const int64_t n = cosmo_system_dim(s);
   const std::string eef = sysname + ".HH.EE";

@@ -33,28 +33,42 @@ namespace cosmo {
 
 __constant__ uint32_t c_binom[33][33];
 
+// Device and stream are per host thread: independent solves driven from several threads
+// (batches, the server's workers) each run on their own default-priority stream and
+// overlap on the device.
 static std::mutex g_mu;
-static int g_device = 0;
-static hipStream_t g_stream[64] = {};
+static thread_local int t_device = 0;
+static thread_local hipStream_t t_stream[64] = {};
 static bool g_binom_done[64] = {};
 
 hipStream_t lib_stream() {
-  std::lock_guard<std::mutex> lk(g_mu);
-  int d = g_device;
-  if (!g_stream[d]) {
+  int d = t_device;
+  if (!t_stream[d]) {
     hipSetDevice(d);
-    hipStreamCreateWithFlags(&g_stream[d], hipStreamNonBlocking);
+    hipStreamCreateWithFlags(&t_stream[d], hipStreamNonBlocking);
   }
-  return g_stream[d];
+  return t_stream[d];
+}
+
+void release_thread_streams() {
+  for (int d = 0; d < 64; ++d)
+    if (t_stream[d]) {
+      hipSetDevice(d);
+      hipStreamSynchronize(t_stream[d]);
+      hipStreamDestroy(t_stream[d]);
+      t_stream[d] = nullptr;
+    }
 }
 
 static int ensure_binom() {
-  int d = g_device;
+  std::lock_guard<std::mutex> lk(g_mu);
+  int d = t_device;
   if (g_binom_done[d]) return COSMO_OK;
   uint32_t b[33][33];
   for (int n = 0; n <= 32; ++n)
     for (int k = 0; k <= 32; ++k) b[n][k] = (uint32_t)binom(n, k);
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), b, sizeof(b)));
+  HIPCHK(hipDeviceSynchronize());
   g_binom_done[d] = true;
   return COSMO_OK;
 }
@@ -902,12 +916,11 @@ int set_device_impl(int d) {
   HIPCHK(hipGetDeviceCount(&cnt));
   if (d < 0 || d >= cnt || d >= 64) return fail(COSMO_EINVAL, "cosmo_set_device: no such device");
   HIPCHK(hipSetDevice(d));
-  std::lock_guard<std::mutex> lk(g_mu);
-  g_device = d;
+  t_device = d;
   return COSMO_OK;
 }
 
-int current_device() { return g_device; }
+int current_device() { return t_device; }
 
 }  // namespace cosmo
 
@@ -936,6 +949,46 @@ int virtual_diag(const DevTables &T, int64_t n, double *d_diag, hipStream_t st) 
   if (n <= 0) return COSMO_OK;
   hipLaunchKernelGGL(k_diag_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, n, d_diag);
   HIPCHK(hipGetLastError());
+  return COSMO_OK;
+}
+}  // namespace cosmo
+
+namespace cosmo {
+// exact device bytes of a stored operator: the builder's count pass without the fill
+int stored_footprint(const System &s, int flags, int64_t *bytes) {
+  hipStream_t st = lib_stream();
+  DevTablesOwner tab;
+  int rc = upload_tables(s, tab, st);
+  if (rc) return rc;
+  const int64_t n = s.dim;
+  int32_t *counts = nullptr;
+  int *err = nullptr;
+  CsrDev M;
+  if (hipMalloc(&counts, sizeof(int32_t) * (n + 1)) != hipSuccess || hipMalloc(&err, sizeof(int)) != hipSuccess) {
+    hipFree(counts);
+    free_tables(tab);
+    return fail(COSMO_ENOMEM, "footprint count buffers");
+  }
+  hipMemsetAsync(err, 0, sizeof(int), st);
+  const int keep = (flags & COSMO_OP_KEEP_SMALL) ? 1 : 0;
+  launch_build<2048>(tab.t, n, 0, keep, M, counts, err, st);
+  int herr = 0;
+  hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
+  hipStreamSynchronize(st);
+  if (herr > 0) {
+    hipMemsetAsync(err, 0, sizeof(int), st);
+    launch_build<8192>(tab.t, n, 0, keep, M, counts, err, st);
+  }
+  std::vector<int32_t> hc(n);
+  hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+  hipError_t e = hipStreamSynchronize(st);
+  hipFree(counts);
+  hipFree(err);
+  int64_t nnz = 0;
+  for (int64_t r = 0; r < n; ++r) nnz += hc[r];
+  *bytes = (int64_t)tab.bytes + 8 * (n + 1) + 8 * n + nnz * (4 + 8 + ((flags & COSMO_OP_NO_FP32) ? 0 : 4));
+  free_tables(tab);
+  if (e != hipSuccess) return fail(COSMO_EHIP, hipGetErrorString(e));
   return COSMO_OK;
 }
 }  // namespace cosmo

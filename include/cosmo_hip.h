@@ -122,6 +122,9 @@ int cosmo_op_destroy(cosmo_op op);
 int64_t cosmo_op_size(cosmo_op op);
 /* Stored upper-triangle and full entry counts (0 for a virtual operator). */
 int cosmo_op_nnz(cosmo_op op, int64_t *nnz_upper, int64_t *nnz_full);
+/* Exact device bytes a stored operator of s would take (the builder's counting pass on
+ * the GPU, without the fill); used by the server's admission control. */
+int cosmo_op_footprint(cosmo_system s, int flags, int64_t *bytes);
 /* Device bytes held by the operator. */
 int64_t cosmo_op_bytes(cosmo_op op);
 /* MatrixBase::Diagonal / SparseMatrixCSB::ReadDiagonal (include/tav/csbmatrix.cxx:393-419). */

@@ -70,18 +70,19 @@ def test_dexhvjtcsb_states(gpu, golden, tmp_path, limit):
         assert abs(x - J * (J + 1)) < 1e-6  # M = 0: <J-J+> = J(J+1)
 
 
-def test_labels_odd_system(gpu, golden):
-    g = golden("sd3p1n_m1_s7")
+@pytest.mark.parametrize("name,twom,twotz", [("sd3p1n_m1_s7", 2, -2), ("fp3p2n_m1_s3", 1, -1)])
+def test_labels_integrality(gpu, golden, name, twom, twotz):
+    """<J-J+> = J(J+1) - M(M+1) with 2J of the parity of A; same for isospin"""
+    g = golden(name)
     s = gpu.System.from_xml(g.xml)
     op = gpu.Operator.build(s)
-    ev, de, rs, vec, st = gpu.davidson(op, 6)
+    ev, de, rs, vec, st = gpu.davidson(op, 6, lim=40)
     jj, tt = s.labels(vec)
     lib = gpu.lib()
-    for x in jj:  # M = 1/2: <J-J+> = J(J+1) - M(M+1) with J half-integer
-        J2 = lib.cosmo_twice_spin(float(x), 1)
-        J = J2 / 2
-        assert J2 % 2 == 1 and abs(x - (J * (J + 1) - 0.75)) < 1e-6
-    for x in tt:  # Tz = (Nv - Zv)/2 = -1
-        T2 = lib.cosmo_twice_spin(float(x), -2)
-        T = T2 / 2
-        assert abs(x - (T * (T + 1) - (-1) * 0)) < 1e-6
+    for x in jj:
+        J2 = lib.cosmo_twice_spin(float(x), twom)
+        assert (J2 - twom) % 2 == 0 and J2 >= abs(twom)
+        assert abs(x - (J2 * (J2 + 2) - twom * (twom + 2)) / 4.0) < 1e-6
+    for x in tt:
+        T2 = lib.cosmo_twice_spin(float(x), twotz)
+        assert abs(x - (T2 * (T2 + 2) - twotz * (twotz + 2)) / 4.0) < 1e-6
