@@ -459,16 +459,16 @@ __global__ __launch_bounds__(256) void k_spmm_vec(int64_t n, const int64_t *__re
   uint32_t c = 0;
   T h = 0;
   if (s + lane < e) {
-    c = cols[s + lane];
-    h = vals[s + lane];
+    c = __builtin_nontemporal_load(cols + s + lane);
+    h = __builtin_nontemporal_load(vals + s + lane);
   }
   for (int64_t base = s; base < e; base += 64) {
     const int64_t nidx = base + 64 + lane;
     uint32_t cn = 0;
     T hn = 0;
     if (nidx < e) {
-      cn = cols[nidx];
-      hn = vals[nidx];
+      cn = __builtin_nontemporal_load(cols + nidx);
+      hn = __builtin_nontemporal_load(vals + nidx);
     }
     const int cnt = (int)min<int64_t>(64, e - base);
 #pragma unroll
