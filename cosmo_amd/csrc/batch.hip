@@ -45,9 +45,12 @@ static int solve_one(System *s, int flags, const cosmo_dav_params &p0, int nume,
   const int64_t n = s->dim;
   p.nume = std::min<int64_t>(nume, n);
   if (p.mblock <= 0 || p.mblock > p.nume) p.mblock = p.nume;
-  int lim = p.lim > 0 ? p.lim : p.nume + 20;
+  // a wider basis than the single-solve default keeps small, nearly degenerate spectra
+  // from stalling; iteration cap raised accordingly
+  int lim = std::max(p.lim, p.nume + std::max(20, p.nume));
   if (lim > n) lim = (int)n;
   p.lim = lim;
+  p.maxiter = std::max(p.maxiter, 4000);
   if (tol > 0) p.critr = tol;
   std::vector<double> de(p.nume);
   rc = cosmo_davidson(op, &p, nullptr, evals, de.data(), resid, nullptr, st);

@@ -632,6 +632,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     ++nloops;
     return COSMO_OK;
   };
+  for (int verify = 0;; ++verify) {
   while (true) {
     for (int i = 0; i < nume; ++i) old[i] = eigs[i];
     diagonalize_S(s, kpass, nume, eigs, svec);
@@ -819,6 +820,23 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     rc = gram(D, D.R, nume, D.R, nume, G);
     if (rc) return rc;
     for (int i = 0; i < nume; ++i) rnorm[i] = std::sqrt(std::max(G[(size_t)i * nume + i], 0.0));
+  }
+  // strict: a pair accepted on an earlier residual may have drifted above critr while the
+  // basis kept growing; reopen such pairs and iterate again from the Ritz vectors
+  bool reopen = false;
+  if (p.strict && fp64_phase && nloops < p.maxiter && ierr == 0 && verify < 20)
+    for (int i = 0; i < nume; ++i)
+      if (rnorm[i] > p.critr) {
+        icv[i] = 0;
+        reopen = true;
+      }
+  if (!reopen) break;
+  std::fill(s.begin(), s.end(), 0.0);
+  for (int i = 0; i < nume; ++i) s[(size_t)i * (i + 1) / 2 + i] = eigs[i];
+  kpass = nume;
+  nncv = nume;
+  omin_to = false;
+  first = true;
   }
   for (int i = 0; i < nume; ++i) {
     evals[i] = eigs[i];

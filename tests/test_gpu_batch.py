@@ -35,7 +35,7 @@ def test_batch_equals_alone(gpu):
     assert st["converged"]
     for s, k, e, r in zip(systems, nume, evb, rsb):
         op = gpu.Operator.build(s)
-        ea, _, ra, _, _ = gpu.davidson(op, k, want_vectors=False, critr=1e-8)
+        ea, _, ra, _, _ = gpu.davidson(op, k, want_vectors=False, critr=1e-8, lim=k + max(20, k), maxiter=4000)
         assert np.array_equal(e, ea), "batched result differs from the solve alone"
     # spot-check against dense eigensolves
     for s, k, e in zip(systems, nume, evb):
