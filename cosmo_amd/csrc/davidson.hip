@@ -30,6 +30,7 @@
 #include "op.h"
 
 namespace cosmo {
+bool dense_symmetric_eigen(int n, const std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
 
 #define GRAM_ROWS 32
 #define GRAM_MAXW 160  // p + q limit of one Gram launch
@@ -498,8 +499,15 @@ static void diagonalize_S(const std::vector<double> &s, int k, int nume, std::ve
   std::vector<double> A((size_t)k * k);
   for (int i = 0; i < k; ++i)
     for (int j = 0; j <= i; ++j) A[i * k + j] = A[j * k + i] = s[(size_t)i * (i + 1) / 2 + j];
-  std::vector<double> w, V;
-  sym_eig(k, A, w, V);
+  // Householder + implicit QL (O(k^3), k <= 128); cyclic Jacobi only if QL fails
+  std::vector<double> w, V, Vc;
+  if (dense_symmetric_eigen(k, A, w, Vc)) {
+    V.resize((size_t)k * k);
+    for (int e = 0; e < k; ++e)
+      for (int r = 0; r < k; ++r) V[(size_t)e * k + r] = Vc[(size_t)r * k + e];
+  } else {
+    sym_eig(k, A, w, V);
+  }
   for (int i = 0; i < k; ++i) eigs[i] = w[i];
   svec.assign(nume, std::vector<double>(k, 0.0));
   for (int e = 0; e < nume && e < k; ++e)
@@ -833,6 +841,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   if (!reopen) break;
   std::fill(s.begin(), s.end(), 0.0);
   for (int i = 0; i < nume; ++i) s[(size_t)i * (i + 1) / 2 + i] = eigs[i];
+  for (int i = 0; i < nume; ++i) eigs[i] = 1.e+30;  // the eigenvalue-change test must not fire
   kpass = nume;
   nncv = nume;
   omin_to = false;

@@ -505,95 +505,6 @@ __global__ __launch_bounds__(256) void k_spmm_vec(int64_t n, const int64_t *__re
   }
 }
 
-// Stored H.X, neutron-block tiled form.  Rows of one neutron configuration b (all its
-// proton states, S <= a few hundred rows) belong to one workgroup.  Each row's entries
-// are sorted by column, hence grouped by the neutron block of the column.  The workgroup
-// walks the column blocks in ascending order in lockstep: the next block is the minimum
-// over its rows' next entries; that block's X rows (S' x ld values, the "X tile") are
-// staged into LDS with coalesced 16-byte loads; every lane group then consumes its rows'
-// entries of that block from the tile.  X traffic drops from one gathered X row per
-// entry to one tile per (row block, column block) pair.
-template <typename T, int LPE, int RMAX>
-__global__ __launch_bounds__(1024) void k_spmm_nb(int nblk, const int64_t *__restrict__ noff,
-                                                 const int32_t *__restrict__ blk_of_row,
-                                                 const int64_t *__restrict__ rowptr,
-                                                 const uint32_t *__restrict__ cols, const T *__restrict__ vals,
-                                                 const T *__restrict__ X, T *__restrict__ Y, int m, int ld,
-                                                 int remap) {
-  using V = typename Vec16<T>::type;
-  constexpr int VPL = Vec16<T>::N;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V *tile = reinterpret_cast<V *>(smem_raw);
-  __shared__ int s_red[16];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-  const int g = tid / LPE, sub = tid % LPE, G = blockDim.x / LPE;
-  const int b = (int)xcd_row_block(remap);
-  if (b >= nblk) return;
-  const int64_t r0 = noff[b];
-  const int S = (int)(noff[b + 1] - r0);
-  int64_t ptr[RMAX], end[RMAX];
-  T acc[RMAX][VPL];
-#pragma unroll
-  for (int i = 0; i < RMAX; ++i) {
-    const int r = g + i * G;
-    ptr[i] = end[i] = 0;
-    if (r < S) {
-      ptr[i] = rowptr[r0 + r];
-      end[i] = rowptr[r0 + r + 1];
-    }
-#pragma unroll
-    for (int q = 0; q < VPL; ++q) acc[i][q] = 0;
-  }
-  for (;;) {
-    int mn = 0x7fffffff;
-#pragma unroll
-    for (int i = 0; i < RMAX; ++i)
-      if (ptr[i] < end[i]) mn = min(mn, blk_of_row[cols[ptr[i]]]);
-    for (int off = 32; off > 0; off >>= 1) mn = min(mn, __shfl_xor(mn, off));
-    if (lane == 0) s_red[wave] = mn;
-    __syncthreads();
-    mn = s_red[0];
-    for (int w = 1; w < nwaves; ++w) mn = min(mn, s_red[w]);
-    if (mn == 0x7fffffff) break;
-    const int64_t c0 = noff[mn], c1 = noff[mn + 1];
-    const int np = (int)(c1 - c0) * LPE;
-    for (int t = tid; t < np; t += blockDim.x)
-      tile[t] = *reinterpret_cast<const V *>(X + (c0 + t / LPE) * ld + (t % LPE) * VPL);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < RMAX; ++i) {
-      while (ptr[i] < end[i]) {
-        const uint32_t c = cols[ptr[i]];
-        if ((int64_t)c >= c1) break;
-        const T h = vals[ptr[i]];
-        const V xv = tile[(c - c0) * LPE + sub];
-        const T *xs = reinterpret_cast<const T *>(&xv);
-#pragma unroll
-        for (int q = 0; q < VPL; ++q) acc[i][q] += h * xs[q];
-        ++ptr[i];
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < RMAX; ++i) {
-    const int r = g + i * G;
-    if (r >= S) continue;
-    T *y = Y + (r0 + r) * ld + sub * VPL;
-    if ((sub + 1) * VPL <= m) {
-      V o;
-      T *os = reinterpret_cast<T *>(&o);
-#pragma unroll
-      for (int q = 0; q < VPL; ++q) os[q] = acc[i][q];
-      *reinterpret_cast<V *>(y) = o;
-    } else {
-#pragma unroll
-      for (int q = 0; q < VPL; ++q)
-        if (sub * VPL + q < m) y[q] = acc[i][q];
-    }
-  }
-}
-
 // FP64 product summed per output element over the full row in ascending column order,
 // one multiply then one add per entry (no contraction): the summation order of
 // SparseMatrixCSB::TimesMatrix for a symmetric matrix (include/tav/csbmatrix.cxx:623-663,
@@ -880,20 +791,6 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
   hipFree(err);
   // upper-triangle count: every row holds its diagonal, off-diagonals come in pairs
   M.nnz_upper = (nnz + n) / 2;
-  // neutron-block structure for the block-tiled H.X
-  M.nblk = (int)s.nmask.size();
-  std::vector<int32_t> bor(n);
-  int maxS = 0;
-  for (int b = 0; b < M.nblk; ++b) {
-    maxS = std::max<int>(maxS, (int)(s.noff[b + 1] - s.noff[b]));
-    for (int64_t r = s.noff[b]; r < s.noff[b + 1]; ++r) bor[r] = b;
-  }
-  M.maxS = maxS;
-  HIPCHK(hipMalloc(&M.noff, sizeof(int64_t) * (M.nblk + 1)));
-  HIPCHK(hipMalloc(&M.blk_of_row, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-  HIPCHK(hipMemcpyAsync(M.noff, s.noff.data(), sizeof(int64_t) * (M.nblk + 1), hipMemcpyHostToDevice, st));
-  if (n) HIPCHK(hipMemcpyAsync(M.blk_of_row, bor.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
   return COSMO_OK;
 }
 
@@ -922,79 +819,13 @@ static bool launch_spmm_vec(const CsrDev &M, const T *vals, const T *X, T *Y, in
   return true;
 }
 
-// block-tiled kernel when the operator carries its neutron-block structure and the
-// block sizes fit (rows per lane group <= 8, X tile <= 96 KB of LDS)
-template <typename T>
-static bool launch_spmm_nb(const CsrDev &M, const T *vals, const T *X, T *Y, int m, int ld, hipStream_t st) {
-  constexpr int VPL = 16 / sizeof(T);
-  if (!M.noff || ld < VPL || (ld & (ld - 1)) || ld > 64 || m > ld) return false;
-  if ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y)) & 15) return false;
-  const int lpe = ld / VPL;
-  const int threads = 1024, G = threads / lpe;
-  const size_t smem = (size_t)M.maxS * ld * sizeof(T);
-  if (M.maxS > G * 8 || smem > 96 * 1024) return false;
-  dim3 g((unsigned)M.nblk), b(threads);
-  const int rm = g_spmm_remap;
-#define NB_CASE(L)                                                                                           \
-  case L:                                                                                                    \
-    hipLaunchKernelGGL((k_spmm_nb<T, L, 8>), g, b, smem, st, M.nblk, M.noff, M.blk_of_row, M.rowptr, M.cols, \
-                       vals, X, Y, m, ld, rm);                                                               \
-    break;
-  switch (lpe) {
-    NB_CASE(1)
-    NB_CASE(2)
-    NB_CASE(4)
-    NB_CASE(8)
-    NB_CASE(16)
-    default: return false;
-  }
-#undef NB_CASE
-  return true;
-}
-
-static int ld_slot(int ld) {
-  int s = 0;
-  while ((1 << s) < ld && s < 7) ++s;
-  return s;
-}
-
 template <typename VT, typename XT, typename AT>
-static int launch_spmm_t(CsrDev &M, const VT *vals, const XT *X, XT *Y, int m, int ldx, int ldy,
+static int launch_spmm_t(const CsrDev &M, const VT *vals, const XT *X, XT *Y, int m, int ldx, int ldy,
                          hipStream_t st) {
   if constexpr (std::is_same<VT, XT>::value) {
-    if (ldx == ldy) {
-      // first use of this (precision, ld): time both kernels, keep the faster (autotune)
-      int &choice = M.choice[sizeof(XT) == 8][ld_slot(ldx)];
-      static int forced = -1;
-      if (forced < 0) forced = getenv("COSMO_SPMM_KERNEL") ? atoi(getenv("COSMO_SPMM_KERNEL")) : 0;
-      if (forced) choice = forced;
-      // small operators (the batched and served solves) use the row kernel: a fixed rule
-      // keeps every solve's bits independent of timing noise
-      if (choice == 0 && M.n < 500000) choice = 1;
-      if (choice == 0) {
-        hipEvent_t e[3];
-        for (auto &x : e) hipEventCreate(&x);
-        float t_row = 1e30f, t_nb = 1e30f;
-        hipEventRecord(e[0], st);
-        bool ok_row = launch_spmm_vec<XT>(M, vals, X, Y, m, ldx, st);
-        hipEventRecord(e[1], st);
-        bool ok_nb = launch_spmm_nb<XT>(M, vals, X, Y, m, ldx, st);
-        hipEventRecord(e[2], st);
-        HIPCHK(hipEventSynchronize(e[2]));
-        if (ok_row) hipEventElapsedTime(&t_row, e[0], e[1]);
-        if (ok_nb) hipEventElapsedTime(&t_nb, e[1], e[2]);
-        for (auto &x : e) hipEventDestroy(x);
-        choice = (ok_nb && t_nb < t_row) ? 2 : 1;
-        if (!ok_row && !ok_nb) choice = 3;  // neither vectorised form applies
-      }
-      if (choice == 2 && launch_spmm_nb<XT>(M, vals, X, Y, m, ldx, st)) {
-        HIPCHK(hipGetLastError());
-        return COSMO_OK;
-      }
-      if (choice != 3 && launch_spmm_vec<XT>(M, vals, X, Y, m, ldx, st)) {
-        HIPCHK(hipGetLastError());
-        return COSMO_OK;
-      }
+    if (ldx == ldy && launch_spmm_vec<XT>(M, vals, X, Y, m, ldx, st)) {
+      HIPCHK(hipGetLastError());
+      return COSMO_OK;
     }
   }
   dim3 g((unsigned)((M.n + 3) / 4)), b(256);
@@ -1043,7 +874,7 @@ int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int pr
     }
     return COSMO_OK;
   }
-  CsrDev &M = op.csr;
+  const CsrDev &M = op.csr;
   for (int v0 = 0; v0 < m; v0 += 64) {
     int mm = std::min(64, m - v0);
     int rc = COSMO_OK;

@@ -12,14 +12,6 @@ struct CsrDev {
   double *v64 = nullptr;
   float *v32 = nullptr;
   double *diag = nullptr;
-  // neutron-block structure of the rows (operators built from a system only): rows of
-  // block b are [noff[b], noff[b+1]); used by the block-tiled H.X kernel
-  int nblk = 0, maxS = 0;
-  int64_t *noff = nullptr;
-  int32_t *blk_of_row = nullptr;
-  // kernel choice per (precision, leading dimension), decided by timing on first use:
-  // 0 unknown, 1 row kernel, 2 block-tiled kernel
-  int choice[2][8] = {};
 };
 
 struct Op {

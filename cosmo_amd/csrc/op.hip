@@ -22,8 +22,6 @@ static void free_op(Op *op) {
   hipFree(M.v64);
   hipFree(M.v32);
   hipFree(M.diag);
-  hipFree(M.noff);
-  hipFree(M.blk_of_row);
   free_tables(op->tab);
   hipFree(op->sx);
   hipFree(op->sy);
