@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libcosmo_hip.so")
 OK = 0
 EINVAL, ENOMEM, EHIP, EIO, EUNSUP, ECONV = -1, -2, -3, -4, -5, -6
 
-OP_STORED, OP_VIRTUAL, OP_KEEP_SMALL, OP_NO_FP32 = 0x0, 0x1, 0x2, 0x4
+OP_STORED, OP_VIRTUAL, OP_KEEP_SMALL, OP_NO_FP32, OP_FACTORED = 0x0, 0x1, 0x2, 0x4, 0x8
 PREC_FP64, PREC_FP32, PREC_FP64_ORDERED = 0, 1, 2
 
 

@@ -588,6 +588,13 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     niv = std::min(p.ninit, lim);
     for (int j = 0; j < niv; ++j)
       HIPCHK(hipMemcpyAsync(D.col(D.B, j), init + (int64_t)j * n, sizeof(double) * n, hipMemcpyHostToDevice, D.st));
+    // caller vectors need not be orthonormal (Ritz vectors from an .EE file already are,
+    // and stay as they are up to rounding); dependent ones are dropped
+    bool rs;
+    int kept;
+    rc = orthonormalise(D, 0, D.col(D.B, 0), niv, 1e-9, rs, kept);
+    if (rc) return rc;
+    niv = kept;
   }
   if (niv < nume) {
     // unit vectors at the lowest diagonal elements; ties keep the lower index
@@ -598,16 +605,17 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
       return dg[a] < dg[b] || (dg[a] == dg[b] && a < b);
     });
     const double one = 1.0;
+    HIPCHK(hipMemsetAsync(D.col(D.B, niv), 0, sizeof(double) * n * (nume - niv), D.st));
     for (int j = niv; j < nume; ++j)
       HIPCHK(hipMemcpyAsync(D.col(D.B, j) + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, D.st));
     HIPCHK(hipStreamSynchronize(D.st));
+    int added = nume - niv;
     if (niv > 0) {
       bool rs;
-      int kept;
-      rc = orthonormalise(D, niv, D.col(D.B, niv), nume - niv, 1e-9, rs, kept);
+      rc = orthonormalise(D, niv, D.col(D.B, niv), nume - niv, 1e-9, rs, added);
       if (rc) return rc;
     }
-    niv = nume;
+    niv += added;
   }
   int prec = mixed ? COSMO_PREC_FP32 : COSMO_PREC_FP64;
   rc = apply_block(D, 0, niv, prec);

@@ -1,0 +1,742 @@
+// factored.hip -- virtual H.X in proton-neutron product form (no H stored).
+//
+// The basis is blocked by neutron configuration nu; the rows of a block are the proton
+// configurations pi of one (M, parity) proton sector.  With E_ca = a+_c a_a,
+//   H = Hn (x) 1 + 1 (x) Hp + sum_{a,c,b,d} W[a][c][b][d] E^n_ca E^p_db,
+// where Hn / Hp hold every one- and two-body term of one species and W the pn two-body
+// terms.  Neutron orbitals are the low bits of a state, so the fermion sign of a pn move
+// is the product of a neutron sign (popcounts inside nu) and a proton sign (inside pi).
+// For block nu the product becomes
+//   Y_nu = sum_nu' Hn[nu,nu'] X_nu' + Hp_s X_nu                      (GEMMs, rocBLAS)
+//        + sum_{a in nu} W_aa . E^p X_nu                               (self step)
+//        + sum_{a->c} phase_n(nu,a,c) sum_{b,d} W_ac[b,d] E^p_db X_nu' (one step per hop)
+// with nu' = nu - a + c.  Each step stages the X rows of block nu' (contiguous: one tile)
+// in LDS and applies, per row pi, the proton hops b -> d whose (delta m, delta parity)
+// matches the neutron hop -- the proton hop lists depend only on the sector and that key,
+// are L2-resident, and are staged in LDS once per key (steps are sorted by key).  One
+// workgroup owns one block (and one column slice of the vectors); its lane groups own
+// rows and keep the sums in registers; the next step's tile is loaded into registers
+// while the current one is consumed.  The single-species parts are dense products per
+// (M_n, parity) class: X is gathered class-contiguous (each class is then an S*ld x nb
+// matrix, one column per block), multiplied by Hn_class and, block by block, by the
+// sector's Hp with rocBLAS; the step kernel adds that result in its epilogue.
+//
+// Hn, Hp and W are produced by the same row generator as the stored operator (the
+// species-only subsystems built with the GPU builder for Hn and Hp; W is the pn part of
+// the forward two-body table as the builder reads it, include/sm/spherical2sp.cxx:441-442
+// ordering), so the product equals the stored operator's up to the summation order.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "op.h"
+
+namespace cosmo {
+
+namespace {
+
+constexpr int FACT_THREADS = 512;
+
+template <typename T>
+struct Piece16;
+template <>
+struct Piece16<float> {
+  using type = float4;
+  static constexpr int N = 4;
+};
+template <>
+struct Piece16<double> {
+  using type = double2;
+  static constexpr int N = 2;
+};
+
+// step code: a | c << 5 | phase_n << 10 | self << 11 | key << 12
+struct FactArgs {
+  int nblk, nn, np2, NK;
+  int hop_regs;  // every (sector, key) hop list fits the register prefetch
+  int wneg;      // offset (in elements) of the negated W slice behind the W slice
+  // LDS byte offsets (the W slice and its negation sit at 0): X tile, step table, key
+  // table, hop row pointers, hop words
+  int lds_tile, lds_st, lds_key, lds_hptr, lds_hw;
+  const int64_t *noff;          // [nblk+1] first row of each block
+  const int32_t *nsector;       // [nblk]
+  const uint32_t *nmask;        // [nblk]
+  const int64_t *poff;          // [nblk] first row in the class-contiguous layout
+  const int32_t *step_start;    // [nblk+1]
+  const int32_t *step_tb;       // target block of each step
+  const uint32_t *step_code;
+  const int64_t *hk_pbeg;       // [nsec*NK] start of the row pointers of (sector, key)
+  const int32_t *hk_ptr;        // row pointers into hk_words
+  const uint32_t *hk_words;     // target row << 16 | 4 * (b*np + d + sign * wneg)
+};
+
+struct StepInfo {
+  uint32_t code;
+  uint32_t t0;  // first row of the target block
+  int S;        // rows of the target block
+  int pad;
+};
+
+struct KeyInfo {
+  int pb;    // row pointers of (sector, key) start at hk_ptr[pb]
+  int base;  // first hop word
+  int cnt;   // hop words
+  int pad;
+};
+
+constexpr int FACT_HOP_REGS = 12;  // hop words per thread held for the next key
+
+// workgroups are dealt round-robin over the 8 XCDs: give each XCD a contiguous range
+// of blocks so that neighbouring blocks share its L2
+__device__ __forceinline__ int xcd_block(int nblk) {
+  const int i = blockIdx.x, q = nblk / 8, r = nblk % 8, x = i % 8;
+  return x * q + (x < r ? x : r) + i / 8;
+}
+
+// pn part of Y for one block and one column slice (the neutron-only and proton-only
+// products are in Yp, added in the epilogue).  Per step the LDS holds the X tile of the
+// target block, the signed W slice of the neutron hop and the proton hop lists of the
+// step's key.  The next step's X tile and W slice, and the hop lists of the next key,
+// are loaded into registers while the current step computes; the block's step and key
+// tables are read into LDS once.
+template <typename T, int LPE, int RMAX>
+__global__ __launch_bounds__(FACT_THREADS) void k_fact(FactArgs A, const T *__restrict__ W,
+                                                       const T *__restrict__ X, const T *__restrict__ Yp,
+                                                       T *__restrict__ Y, int ld, int m) {
+  constexpr int VPL = Piece16<T>::N;
+  // native vectors (not HIP's union vector types, which keep a register array in scratch)
+  typedef T NV __attribute__((ext_vector_type(VPL)));
+  constexpr int G = FACT_THREADS / LPE;  // lane groups (one row each per pass)
+  constexpr int SW = LPE * VPL;          // vector columns of one slice
+  constexpr int PP = (RMAX * G + FACT_THREADS) / FACT_THREADS;  // row pointers per thread
+  extern __shared__ __align__(16) unsigned char lds[];
+  T *wsl = reinterpret_cast<T *>(lds);
+  NV *tile = reinterpret_cast<NV *>(lds + A.lds_tile);
+  StepInfo *sst = reinterpret_cast<StepInfo *>(lds + A.lds_st);
+  KeyInfo *skey = reinterpret_cast<KeyInfo *>(lds + A.lds_key);
+  int *hptr = reinterpret_cast<int *>(lds + A.lds_hptr);
+  uint32_t *hw = reinterpret_cast<uint32_t *>(lds + A.lds_hw);
+
+  const int blk = xcd_block(A.nblk);
+  const int col0 = blockIdx.y * SW;
+  if (col0 >= m) return;
+  const int tid = threadIdx.x, g = tid / LPE, sub = tid % LPE;
+  const int64_t r0 = A.noff[blk];
+  const int S = (int)(A.noff[blk + 1] - r0);
+  const int sec = A.nsector[blk];
+  const uint32_t nu = A.nmask[blk];
+  const int st0 = A.step_start[blk], nst = A.step_start[blk + 1] - st0;
+
+  // block header: step table and key table of the sector
+  for (int j = tid; j < nst; j += FACT_THREADS) {
+    const int tb = A.step_tb[st0 + j];
+    const int64_t t0 = A.noff[tb];
+    sst[j] = StepInfo{A.step_code[st0 + j], (uint32_t)t0, (int)(A.noff[tb + 1] - t0), 0};
+  }
+  for (int k = tid; k < A.NK; k += FACT_THREADS) {
+    const int pb = (int)A.hk_pbeg[(int64_t)sec * A.NK + k];
+    const int base = A.hk_ptr[pb];
+    skey[k] = KeyInfo{pb, base, A.hk_ptr[pb + S] - base, 0};
+  }
+  __syncthreads();
+
+  T acc[RMAX][VPL];
+#pragma unroll
+  for (int i = 0; i < RMAX; ++i)
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) acc[i][v] = 0;
+
+  NV pre[RMAX];
+  T wp[2];
+  uint32_t hreg[FACT_HOP_REGS];
+  int preg[PP];
+  // Prefetch loads are unconditional (addresses clamped into range, surplus lanes load a
+  // duplicate): a load under a divergent branch is waited for at the end of that branch,
+  // which would serialise the prefetch instead of overlapping it with the step's compute.
+  auto prefetch = [&](const StepInfo &si) {
+    const int npc = si.S * LPE;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j) {
+      const int p = min(tid + j * FACT_THREADS, npc - 1);
+      pre[j] = *reinterpret_cast<const NV *>(X + ((int64_t)si.t0 + p / LPE) * ld + col0 + (p % LPE) * VPL);
+    }
+    if (!((si.code >> 11) & 1u)) {
+      const T *src = W + (int64_t)((si.code & 31) * A.nn + ((si.code >> 5) & 31)) * A.np2;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) wp[j] = src[min(tid + j * FACT_THREADS, A.np2 - 1)];
+    } else {  // the block itself: W summed over its occupied neutron orbitals a (a -> a)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int t = min(tid + j * FACT_THREADS, A.np2 - 1);
+        T w = 0;
+        for (uint32_t bb = nu; bb; bb &= bb - 1) {
+          const int a = __ffs(bb) - 1;
+          w += W[(int64_t)(a * A.nn + a) * A.np2 + t];
+        }
+        wp[j] = w;
+      }
+    }
+  };
+  auto hop_prefetch = [&](int key) {
+    const KeyInfo ki = skey[key];
+#pragma unroll
+    for (int j = 0; j < PP; ++j) preg[j] = A.hk_ptr[ki.pb + min(tid + j * FACT_THREADS, S)] - ki.base;
+    const int last = max(ki.cnt - 1, 0);  // hk_words holds one spare entry at its end
+#pragma unroll
+    for (int j = 0; j < FACT_HOP_REGS; ++j) hreg[j] = A.hk_words[ki.base + min(tid + j * FACT_THREADS, last)];
+  };
+  int cur_key = -1;
+  prefetch(sst[0]);
+  if (A.hop_regs) hop_prefetch((int)(sst[0].code >> 12));
+  for (int i = 0; i < nst; ++i) {
+    const StepInfo si = sst[i];
+    const int key = (int)(si.code >> 12);
+    const bool self = (si.code >> 11) & 1u;
+    __syncthreads();  // the previous step is done with the LDS
+    {
+      const int npc = si.S * LPE;
+#pragma unroll
+      for (int j = 0; j < RMAX; ++j) {
+        const int p = tid + j * FACT_THREADS;
+        if (p < npc) tile[p] = pre[j];
+      }
+    }
+    {
+      const bool neg = !self && ((si.code >> 10) & 1u);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int t = tid + j * FACT_THREADS;
+        if (t < A.np2) {
+          wsl[t] = neg ? -wp[j] : wp[j];
+          wsl[A.wneg + t] = neg ? wp[j] : -wp[j];
+        }
+      }
+    }
+    if (key != cur_key) {
+      if (A.hop_regs) {
+        const int cnt = skey[key].cnt;
+#pragma unroll
+        for (int j = 0; j < PP; ++j) {
+          const int r = tid + j * FACT_THREADS;
+          if (r <= S) hptr[r] = preg[j];
+        }
+#pragma unroll
+        for (int j = 0; j < FACT_HOP_REGS; ++j) {
+          const int e = tid + j * FACT_THREADS;
+          if (e < cnt) hw[e] = hreg[j];
+        }
+      } else {  // lists beyond the register budget: staged straight from memory
+        const KeyInfo ki = skey[key];
+        for (int r = tid; r <= S; r += FACT_THREADS) hptr[r] = A.hk_ptr[ki.pb + r] - ki.base;
+        for (int e = tid; e < ki.cnt; e += FACT_THREADS) hw[e] = A.hk_words[ki.base + e];
+      }
+      cur_key = key;
+    }
+    __syncthreads();
+    if (i + 1 < nst) prefetch(sst[i + 1]);  // in flight while this step computes
+    if (A.hop_regs && i + 1 < nst) {
+      const int nk = (int)(sst[i + 1].code >> 12);
+      if (nk != key) hop_prefetch(nk);
+    }
+    // hop word w: target row in the high half; in the low half the byte offset of its
+    // (signed) W entry for floats, half of it for doubles -- the W slices start at LDS
+    // address 0, so the offset is the address
+    typedef const __attribute__((address_space(3))) unsigned char lds_byte;
+    const uint32_t lbase = (uint32_t)(uintptr_t)(lds_byte *)lds;
+    const uint32_t tl = lbase + (uint32_t)A.lds_tile + sub * 16;
+    auto wval = [&](uint32_t w) -> T {
+      return *(const __attribute__((address_space(3))) T *)(lbase + (sizeof(T) == 8 ? (w & 0xffffu) << 1 : (w & 0xffffu)));
+    };
+    auto xrow = [&](uint32_t w) -> NV {
+      return *(const __attribute__((address_space(3))) NV *)(tl + (w >> 16) * (LPE * 16));
+    };
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j) {
+      const int r = g + j * G;
+      if (r < S) {
+        int e = hptr[r];
+        const int e1 = hptr[r + 1];
+        for (; e + 4 <= e1; e += 4) {
+          uint32_t w[4];
+          T h[4];
+          NV xv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w[u] = hw[e + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            h[u] = wval(w[u]);
+            xv[u] = xrow(w[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) acc[j][v] += h[u] * xv[u][v];
+        }
+        for (; e < e1; ++e) {
+          const uint32_t w = hw[e];
+          const T h = wval(w);
+          const NV xv = xrow(w);
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) acc[j][v] += h * xv[v];
+        }
+      }
+    }
+  }
+  // epilogue: add the single-species products and store the columns below m
+  const int64_t pr0 = A.poff[blk];
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j) {
+    const int r = g + j * G;
+    if (r < S) {
+      const int c = col0 + sub * VPL;
+      const NV yn = *reinterpret_cast<const NV *>(Yp + (pr0 + r) * ld + c);
+      T *y = Y + (r0 + r) * ld + c;
+      if (c + VPL <= m) {
+        NV o;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) o[v] = acc[j][v] + yn[v];
+        *reinterpret_cast<NV *>(y) = o;
+      } else {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v)
+          if (c + v < m) y[v] = acc[j][v] + yn[v];
+      }
+    }
+  }
+}
+
+// class-contiguous copy of X: block b's rows move to row poff[b]
+template <typename T>
+__global__ __launch_bounds__(256) void k_class_gather(int nblk, const int64_t *__restrict__ noff,
+                                                      const int64_t *__restrict__ poff, const T *__restrict__ X,
+                                                      T *__restrict__ Xp, int ld) {
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int64_t len = (noff[b + 1] - noff[b]) * ld;
+  const T *src = X + noff[b] * ld;
+  T *dst = Xp + poff[b] * ld;
+  for (int64_t i = threadIdx.x; i < len; i += 256) dst[i] = src[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+struct FactOp {
+  FactArgs A{};
+  std::vector<void *> owned;  // device allocations
+  const double *W64 = nullptr;
+  const float *W32 = nullptr;
+  int maxS = 0, hopcap = 0, np2 = 0, maxsteps = 0;
+  // one (M_n, parity) neutron class: nb blocks of S rows, contiguous from row prow of the
+  // class-contiguous layout; dense Hn (nb x nb) and the dense Hp of its proton sector (S x S)
+  struct Cls {
+    int nb = 0, S = 0;
+    int64_t prow = 0;
+    const double *H64 = nullptr, *P64 = nullptr;
+    const float *H32 = nullptr, *P32 = nullptr;
+  };
+  std::vector<Cls> cls;
+  void *Xp = nullptr, *Yp = nullptr;
+  size_t p_bytes = 0;
+  rocblas_handle blas = nullptr;
+  int64_t n = 0;
+  size_t bytes = 0;
+  ~FactOp() {
+    for (void *p : owned) hipFree(p);
+    hipFree(Xp);
+    hipFree(Yp);
+    if (blas) rocblas_destroy_handle(blas);
+  }
+  template <typename T>
+  int put(const std::vector<T> &h, const T **out) {
+    void *d = nullptr;
+    const size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
+    if (hipMalloc(&d, b) != hipSuccess) return fail(COSMO_ENOMEM, "factorised operator: table allocation failed");
+    owned.push_back(d);
+    bytes += b;
+    if (!h.empty() && hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(COSMO_EHIP, "factorised operator: table upload failed");
+    *out = (const T *)d;
+    return COSMO_OK;
+  }
+};
+
+// Full symmetric rows of a species-only subsystem (upper rows from the GPU builder).
+static int subsystem_rows(const System &sub, std::vector<int64_t> &rp, std::vector<int64_t> &cols,
+                          std::vector<double> &vals) {
+  cosmo_system_s tmp;
+  static_cast<System &>(tmp) = sub;
+  cosmo_op op = nullptr;
+  int rc = cosmo_op_create(&tmp, COSMO_OP_NO_FP32 | COSMO_OP_KEEP_SMALL, &op);
+  if (rc) return rc;
+  const int64_t n = cosmo_op_size(op);
+  std::vector<int64_t> cnt(n), up(n + 1, 0);
+  rc = cosmo_op_upper_rows(op, 0, n, cnt.data(), nullptr, nullptr, nullptr);
+  for (int64_t r = 0; r < n && !rc; ++r) up[r + 1] = up[r] + cnt[r];
+  std::vector<int64_t> uc(up[n]);
+  std::vector<double> uv(up[n]);
+  if (!rc) rc = cosmo_op_upper_rows(op, 0, n, cnt.data(), up.data(), uc.data(), uv.data());
+  cosmo_op_destroy(op);
+  if (rc) return rc;
+  std::vector<int64_t> deg(n, 0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = up[r]; e < up[r + 1]; ++e) {
+      ++deg[r];
+      if (uc[e] != r) ++deg[uc[e]];
+    }
+  rp.assign(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r) rp[r + 1] = rp[r] + deg[r];
+  cols.assign(rp[n], 0);
+  vals.assign(rp[n], 0.0);
+  std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = up[r]; e < up[r + 1]; ++e) {
+      const int64_t c = uc[e];
+      cols[fill[r]] = c;
+      vals[fill[r]++] = uv[e];
+      if (c != r) {
+        cols[fill[c]] = r;
+        vals[fill[c]++] = uv[e];
+      }
+    }
+  return COSMO_OK;
+}
+
+static inline int popc(uint32_t x) { return __builtin_popcount(x); }
+static inline uint32_t below32(int i) { return (1u << i) - 1u; }
+
+int build_factored(Op &op, const System &s) {
+  const int nn = s.nn, np = s.np;
+  const int nblk = (int)s.nmask.size();
+  if (s.nv < 1 || s.zv < 1) return fail(COSMO_EUNSUP, "factorised operator needs valence neutrons and protons");
+  if (nn > 31 || np > 31) return fail(COSMO_EUNSUP, "factorised operator: more than 31 orbitals of a species");
+  // every term must keep the neutron and proton counts
+  for (size_t t = 0; t < s.esp_v.size(); ++t)
+    if ((s.esp_k[2 * t] < nn) != (s.esp_k[2 * t + 1] < nn))
+      return fail(COSMO_EUNSUP, "factorised operator: one-body term changes the species");
+  for (size_t t = 0; t < s.vspx_v.size(); ++t) {
+    const uint16_t *k = &s.vspx_k[4 * t];
+    if ((k[0] < nn) + (k[1] < nn) != (k[2] < nn) + (k[3] < nn))
+      return fail(COSMO_EUNSUP, "factorised operator: two-body term changes the species");
+  }
+  auto F = std::make_unique<FactOp>();
+  FactOp &f = *F;
+  f.n = s.dim;
+  f.np2 = np * np;
+  const int nsec = (int)s.psec_start.size() - 1;
+  int maxS = 0;
+  for (int b = 0; b < nblk; ++b) maxS = std::max<int>(maxS, (int)(s.noff[b + 1] - s.noff[b]));
+  if (maxS > 4096) return fail(COSMO_EUNSUP, "factorised operator: proton sector above 4096 states");
+  f.maxS = maxS;
+  // keys: (proton delta 2m, delta parity)
+  int m2max = 1;
+  for (int i = nn; i < nn + np; ++i) m2max = std::max(m2max, std::abs(s.orb[i].m2));
+  const int dmin = -2 * m2max;
+  const int NK = 2 * (2 * m2max + 1);
+  auto key_of = [&](int dm2, int dpar) { return ((dm2 - dmin) / 2) * 2 + dpar; };
+  std::vector<int32_t> psec_of(s.pcfg_rank.size(), -1);
+  for (int sec = 0; sec < nsec; ++sec)
+    for (int64_t q = s.psec_start[sec]; q < s.psec_start[sec + 1]; ++q) psec_of[colex_rank(s.pmask[q])] = sec;
+
+  // ---- proton hop lists per (sector, key), CSR over the sector's rows
+  std::vector<int64_t> hk_pbeg((size_t)nsec * NK);
+  std::vector<int32_t> hk_ptr;
+  std::vector<uint32_t> hk_words;
+  int wneg = 16;  // the negated W slice follows at a power of two >= np*np
+  while (wneg < np * np) wneg <<= 1;
+  int hopcap = 0;
+  for (int sec = 0; sec < nsec; ++sec) {
+    const int64_t qa = s.psec_start[sec], qb = s.psec_start[sec + 1];
+    std::vector<std::vector<std::vector<uint32_t>>> rows(NK, std::vector<std::vector<uint32_t>>(qb - qa));
+    for (int64_t q = qa; q < qb; ++q) {
+      const uint32_t P = s.pmask[q];
+      for (uint32_t bb = P; bb; bb &= bb - 1) {
+        const int b = __builtin_ctz(bb);
+        for (int d = 0; d < np; ++d) {
+          if (d != b && ((P >> d) & 1u)) continue;
+          const uint32_t P2 = (P & ~(1u << b)) | (1u << d);
+          if (psec_of[colex_rank(P2)] < 0) continue;
+          const int t = s.pcfg_rank[colex_rank(P2)];
+          const int sign = (popc(P & below32(b)) + popc((P & ~(1u << b)) & below32(d))) & 1;
+          const Orbital &ob = s.orb[nn + b], &od = s.orb[nn + d];
+          const int key = key_of(od.m2 - ob.m2, ob.par ^ od.par);
+          rows[key][q - qa].push_back((uint32_t)t << 16 | (uint32_t)(4 * (b * np + d + sign * wneg)));
+        }
+      }
+    }
+    for (int k = 0; k < NK; ++k) {
+      hk_pbeg[(size_t)sec * NK + k] = (int64_t)hk_ptr.size();
+      const size_t base = hk_words.size();
+      for (auto &r : rows[k]) {
+        hk_ptr.push_back((int32_t)hk_words.size());
+        hk_words.insert(hk_words.end(), r.begin(), r.end());
+      }
+      hk_ptr.push_back((int32_t)hk_words.size());
+      hopcap = std::max<int>(hopcap, (int)(hk_words.size() - base));
+    }
+  }
+  hk_words.push_back(0);  // spare entry: clamped prefetch reads of an empty last list
+  f.hopcap = hopcap;
+
+  // ---- pn table W[a*nn + c][b*np + d] from the forward two-body keys and their mirrors
+  std::vector<double> W((size_t)nn * nn * f.np2, 0.0);
+  for (size_t t = 0; t < s.vspx_v.size(); ++t) {
+    const uint16_t *k = &s.vspx_k[4 * t];
+    if (k[2] < k[0] || (k[2] == k[0] && k[3] < k[1])) continue;  // not forward: dropped
+    if (!(k[0] < nn && k[1] >= nn)) continue;                     // not a pn pair
+    const int a = k[0], b = k[1] - nn, c = k[2], d = k[3] - nn;
+    W[(size_t)(a * nn + c) * f.np2 + b * np + d] = s.vspx_v[t];
+    W[(size_t)(c * nn + a) * f.np2 + d * np + b] = s.vspx_v[t];
+  }
+  std::vector<uint8_t> wnz((size_t)nn * nn, 0);
+  for (int ac = 0; ac < nn * nn; ++ac)
+    for (int bd = 0; bd < f.np2; ++bd)
+      if (W[(size_t)ac * f.np2 + bd] != 0.0) wnz[ac] = 1;
+
+  // ---- steps per block: the block itself, then every neutron hop a -> c, sorted by key
+  const int key0 = key_of(0, 0);
+  std::vector<int32_t> step_start(nblk + 1, 0), step_tb;
+  std::vector<uint32_t> step_code;
+  std::vector<std::pair<int, std::pair<int32_t, uint32_t>>> st;
+  for (int blk = 0; blk < nblk; ++blk) {
+    const uint32_t nu = s.nmask[blk];
+    st.clear();
+    st.push_back({key0 * 2, {blk, (1u << 11) | (uint32_t)key0 << 12}});
+    for (uint32_t aa = nu; aa; aa &= aa - 1) {
+      const int a = __builtin_ctz(aa);
+      for (int c = 0; c < nn; ++c) {
+        if (((nu >> c) & 1u) || !wnz[a * nn + c]) continue;
+        const uint32_t nu2 = (nu & ~(1u << a)) | (1u << c);
+        const int tb = s.ncfg_index[colex_rank(nu2)];
+        if (tb < 0) continue;
+        const int dm2 = s.orb[a].m2 - s.orb[c].m2;  // the proton side changes 2m by this
+        if (dm2 < dmin || dm2 > -dmin) continue;
+        const int key = key_of(dm2, s.orb[a].par ^ s.orb[c].par);
+        const int ph = (popc(nu & below32(a)) + popc((nu & ~(1u << a)) & below32(c))) & 1;
+        st.push_back({key * 2 + 1, {tb, (uint32_t)a | (uint32_t)c << 5 | (uint32_t)ph << 10 | (uint32_t)key << 12}});
+      }
+    }
+    std::stable_sort(st.begin(), st.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (auto &x : st) {
+      step_tb.push_back(x.second.first);
+      step_code.push_back(x.second.second);
+    }
+    step_start[blk + 1] = (int32_t)step_tb.size();
+    f.maxsteps = std::max<int>(f.maxsteps, (int)st.size());
+  }
+
+  // ---- Hp: dense per proton sector, from the proton-only subsystem (no neutrons)
+  std::vector<std::vector<double>> hp_dense(nsec);
+  size_t dense_bytes = 0;
+  for (int sec = 0; sec < nsec; ++sec) {
+    const size_t S = (size_t)(s.psec_start[sec + 1] - s.psec_start[sec]);
+    dense_bytes += S * S * 12;
+  }
+  for (int sec = 0; sec < nsec; ++sec) {
+    const int64_t qa = s.psec_start[sec], S = s.psec_start[sec + 1] - qa;
+    if (dense_bytes > ((size_t)2 << 30)) break;  // reported below
+    int M = 0, par = 0;
+    for (uint32_t bb = s.pmask[qa]; bb; bb &= bb - 1) {
+      M += s.orb[nn + __builtin_ctz(bb)].m2;
+      par ^= s.orb[nn + __builtin_ctz(bb)].par;
+    }
+    System sub = s;
+    sub.nv = 0;
+    sub.twoM = M;
+    sub.parity = par;
+    sub.build_basis();
+    // the subsystem enumerates the proton configurations exactly as s does: its one block
+    // (no neutrons) must be this sector
+    if (sub.dim != S || sub.nsector.size() != 1 || sub.nsector[0] != sec || sub.pmask != s.pmask)
+      return fail(COSMO_EINVAL, "factorised operator: proton subsystem differs from its sector");
+    std::vector<int64_t> rp, cc;
+    std::vector<double> vv;
+    int rc = subsystem_rows(sub, rp, cc, vv);
+    if (rc) return rc;
+    hp_dense[sec].assign((size_t)S * S, 0.0);
+    for (int64_t r = 0; r < S; ++r)
+      for (int64_t e = rp[r]; e < rp[r + 1]; ++e) hp_dense[sec][(size_t)cc[e] * S + r] = vv[e];
+  }
+
+  // ---- Hn: dense per (M_n, parity) class, and the class-contiguous block order
+  std::map<std::pair<int, int>, std::vector<int>> byclass;
+  for (int b = 0; b < nblk; ++b) {
+    int M = 0, par = 0;
+    for (uint32_t bb = s.nmask[b]; bb; bb &= bb - 1) {
+      M += s.orb[__builtin_ctz(bb)].m2;
+      par ^= s.orb[__builtin_ctz(bb)].par;
+    }
+    byclass[{M, par}].push_back(b);
+  }
+  for (auto &kv : byclass) dense_bytes += kv.second.size() * kv.second.size() * 12;
+  if (dense_bytes > ((size_t)2 << 30))
+    return fail(COSMO_EUNSUP, "factorised operator: single-species blocks too large for dense products");
+  std::vector<int64_t> poff(nblk, 0);
+  int64_t prow = 0;
+  for (auto &kv : byclass) {
+    System sub = s;
+    sub.zv = 0;
+    sub.twoM = kv.first.first;
+    sub.parity = kv.first.second;
+    sub.build_basis();
+    const int N = (int)kv.second.size();
+    bool same = sub.dim == N;
+    for (int i = 0; same && i < N; ++i) same = sub.nmask[i] == s.nmask[kv.second[i]];
+    if (!same) return fail(COSMO_EINVAL, "factorised operator: neutron subsystem differs from its class");
+    std::vector<int64_t> rp, cc;
+    std::vector<double> vv;
+    int rc = subsystem_rows(sub, rp, cc, vv);
+    if (rc) return rc;
+    std::vector<double> Hd((size_t)N * N, 0.0);
+    for (int r = 0; r < N; ++r)
+      for (int64_t e = rp[r]; e < rp[r + 1]; ++e) Hd[(size_t)cc[e] * N + r] = vv[e];
+    std::vector<float> Hf(Hd.begin(), Hd.end());
+    FactOp::Cls c;
+    c.nb = N;
+    c.S = (int)(s.noff[kv.second[0] + 1] - s.noff[kv.second[0]]);
+    c.prow = prow;
+    for (int b : kv.second) {
+      poff[b] = prow;
+      prow += c.S;
+    }
+    const int sec = s.nsector[kv.second[0]];
+    std::vector<float> Pf(hp_dense[sec].begin(), hp_dense[sec].end());
+    if ((rc = f.put(Hd, &c.H64)) || (rc = f.put(Hf, &c.H32)) || (rc = f.put(hp_dense[sec], &c.P64)) ||
+        (rc = f.put(Pf, &c.P32)))
+      return rc;
+    f.cls.push_back(c);
+  }
+
+  // ---- upload
+  std::vector<float> W32(W.begin(), W.end());
+  FactArgs &A = f.A;
+  A.nblk = nblk;
+  A.nn = nn;
+  A.np2 = f.np2;
+  A.NK = NK;
+  A.wneg = wneg;
+  int rc;
+  if ((rc = f.put(s.noff, &A.noff)) || (rc = f.put(s.nsector, &A.nsector)) || (rc = f.put(s.nmask, &A.nmask)) ||
+      (rc = f.put(poff, &A.poff)) || (rc = f.put(step_start, &A.step_start)) || (rc = f.put(step_tb, &A.step_tb)) ||
+      (rc = f.put(step_code, &A.step_code)) || (rc = f.put(hk_pbeg, &A.hk_pbeg)) || (rc = f.put(hk_ptr, &A.hk_ptr)) ||
+      (rc = f.put(hk_words, &A.hk_words)) || (rc = f.put(W, &f.W64)) || (rc = f.put(W32, &f.W32)))
+    return rc;
+  if (rocblas_create_handle(&f.blas) != rocblas_status_success)
+    return fail(COSMO_EHIP, "factorised operator: rocblas_create_handle failed");
+  op.fact = F.release();
+  return COSMO_OK;
+}
+
+void free_factored(Op &op) {
+  delete static_cast<FactOp *>(op.fact);
+  op.fact = nullptr;
+}
+
+size_t factored_bytes(const Op &op) {
+  const FactOp *f = static_cast<const FactOp *>(op.fact);
+  return f ? f->bytes + 2 * f->p_bytes : 0;
+}
+
+template <typename T, int LPE>
+static bool launch_fact_r(int rmax, dim3 g, size_t lds, hipStream_t st, const FactArgs &A, const T *W, const T *X,
+                          const T *Yp, T *Y, int ld, int m) {
+#define FACT_R(R)                                                                                    \
+  if (rmax <= R) {                                                                                   \
+    auto kern = k_fact<T, LPE, R>;                                                                   \
+    if (lds > 65536)                                                                                 \
+      hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    hipLaunchKernelGGL(kern, g, dim3(FACT_THREADS), lds, st, A, W, X, Yp, Y, ld, m);                \
+    return true;                                                                                     \
+  }
+  FACT_R(1)
+  FACT_R(2)
+  FACT_R(3)
+  FACT_R(4)
+  FACT_R(5)
+  FACT_R(6)
+  FACT_R(8)
+  FACT_R(12)
+  FACT_R(16)
+#undef FACT_R
+  return false;
+}
+
+static inline int align16(size_t b) { return (int)((b + 15) & ~size_t(15)); }
+
+template <typename T>
+static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
+  FactOp &f = *static_cast<FactOp *>(op.fact);
+  constexpr int VPL = Piece16<T>::N;
+  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
+    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
+  // slice width: up to 64 bytes of every row, narrower when the tile would not fit
+  int lpe = std::min(4, ld / VPL);
+  while (lpe > 1 && (size_t)f.maxS * lpe * 16 > 64 * 1024) lpe >>= 1;
+  const int G = FACT_THREADS / lpe;
+  const int rmax = (f.maxS + G - 1) / G;
+  const int slices = ld / (lpe * VPL);
+  FactArgs A = f.A;
+  A.lds_tile = align16((size_t)2 * A.wneg * sizeof(T));
+  A.lds_st = A.lds_tile + align16((size_t)f.maxS * lpe * 16);
+  A.lds_key = A.lds_st + align16((size_t)f.maxsteps * sizeof(StepInfo));
+  A.lds_hptr = A.lds_key + align16((size_t)A.NK * sizeof(KeyInfo));
+  A.lds_hw = A.lds_hptr + align16((size_t)(f.maxS + 1) * 4);
+  const size_t lds = (size_t)A.lds_hw + (size_t)f.hopcap * 4;
+  A.hop_regs = f.hopcap <= FACT_HOP_REGS * FACT_THREADS;
+  if (lds > 160 * 1024 || f.np2 > 2 * FACT_THREADS)
+    return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
+  // single-species parts on the class-contiguous copy of X:
+  //   Yp_class = Xp_class Hn_class            (one GEMM, columns = neutron blocks)
+  //   Yp_blk  += Xp_blk Hp_sector per block   (strided batched GEMM, blocks of S rows)
+  const size_t need = (size_t)f.n * ld * sizeof(T);
+  if (f.p_bytes < need) {
+    hipFree(f.Xp);
+    hipFree(f.Yp);
+    f.Xp = f.Yp = nullptr;
+    f.p_bytes = 0;
+    HIPCHK(hipMalloc(&f.Xp, need));
+    HIPCHK(hipMalloc(&f.Yp, need));
+    f.p_bytes = need;
+  }
+  T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp;
+  hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)A.nblk), dim3(256), 0, st, A.nblk, A.noff, A.poff, X, Xp, ld);
+  HIPCHK(hipGetLastError());
+  if (rocblas_set_stream(f.blas, st) != rocblas_status_success) return fail(COSMO_EHIP, "rocblas_set_stream failed");
+  const T one = 1, zero = 0;
+  const auto N_ = rocblas_operation_none;
+  for (auto &c : f.cls) {
+    const rocblas_int L = (rocblas_int)((int64_t)c.S * ld), N = c.nb, S = c.S;
+    T *xc = Xp + c.prow * ld, *yc = Yp + c.prow * ld;
+    rocblas_status b1, b2;
+    if constexpr (sizeof(T) == 8) {
+      b1 = rocblas_dgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H64, N, &zero, yc, L);
+      b2 = rocblas_dgemm_strided_batched(f.blas, N_, N_, ld, S, S, &one, xc, ld, L, c.P64, S, 0, &one, yc, ld, L, N);
+    } else {
+      b1 = rocblas_sgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H32, N, &zero, yc, L);
+      b2 = rocblas_sgemm_strided_batched(f.blas, N_, N_, ld, S, S, &one, xc, ld, L, c.P32, S, 0, &one, yc, ld, L, N);
+    }
+    if (b1 != rocblas_status_success || b2 != rocblas_status_success)
+      return fail(COSMO_EHIP, "factorised apply: rocblas gemm failed");
+  }
+  const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
+  dim3 g((unsigned)A.nblk, (unsigned)slices);
+  bool ok;
+  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
+  else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
+  else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
+  if (!ok) return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
+  HIPCHK(hipGetLastError());
+  return COSMO_OK;
+}
+
+int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st) {
+  if (prec == COSMO_PREC_FP32) return fact_apply_t<float>(op, (const float *)X, (float *)Y, m, ld, st);
+  return fact_apply_t<double>(op, (const double *)X, (double *)Y, m, ld, st);
+}
+
+}  // namespace cosmo

@@ -505,77 +505,6 @@ __global__ __launch_bounds__(256) void k_spmm_vec(int64_t n, const int64_t *__re
   }
 }
 
-// Stored H.X, lane-per-entry form: every lane owns one entry of the 64-entry chunk and
-// gathers its whole X row (LD values, 16-byte pieces) without waiting for a broadcast,
-// so all gathers of a chunk are independent; each lane keeps LD partial sums and the
-// row's result is reduced at the end by halving exchanges (LD values -> 1 per lane).
-template <typename T, int LD>
-__global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t *__restrict__ rowptr,
-                                                   const uint32_t *__restrict__ cols, const T *__restrict__ vals,
-                                                   const T *__restrict__ X, T *__restrict__ Y, int m, int remap) {
-  using V = typename Vec16<T>::type;
-  constexpr int VPL = Vec16<T>::N;
-  constexpr int PIECES = LD / VPL;
-  const int lane = threadIdx.x & 63;
-  const int64_t row = xcd_row_block(remap) * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const int64_t s = rowptr[row], e = rowptr[row + 1];
-  T acc[LD];
-#pragma unroll
-  for (int q = 0; q < LD; ++q) acc[q] = 0;
-  uint32_t c = 0;
-  T h = 0;
-  if (s + lane < e) {
-    c = __builtin_nontemporal_load(cols + s + lane);
-    h = __builtin_nontemporal_load(vals + s + lane);
-  }
-  for (int64_t base = s; base < e; base += 64) {
-    const int64_t nidx = base + 64 + lane;
-    uint32_t cn = 0;
-    T hn = 0;
-    if (nidx < e) {
-      cn = __builtin_nontemporal_load(cols + nidx);
-      hn = __builtin_nontemporal_load(vals + nidx);
-    }
-    if (base + lane < e) {
-      V xv[PIECES];
-      const V *xr = reinterpret_cast<const V *>(X + (int64_t)c * LD);
-#pragma unroll
-      for (int p = 0; p < PIECES; ++p) xv[p] = xr[p];
-#pragma unroll
-      for (int p = 0; p < PIECES; ++p) {
-        const T *xs = reinterpret_cast<const T *>(&xv[p]);
-#pragma unroll
-        for (int q = 0; q < VPL; ++q) acc[p * VPL + q] += h * xs[q];
-      }
-    }
-    c = cn;
-    h = hn;
-  }
-  // halving reduction: after the step with offset OFF each lane keeps half of its values
-  int qsel = 0;
-#define COSMO_HALVE(NH, OFF)                                                      \
-  {                                                                             \
-    const bool up = (lane & (OFF)) != 0;                                        \
-    _Pragma("unroll") for (int q = 0; q < (NH); ++q) {                          \
-      const T send = up ? acc[q] : acc[q + (NH)];                               \
-      const T recv = __shfl_xor(send, (OFF));                                   \
-      acc[q] = (up ? acc[q + (NH)] : acc[q]) + recv;                            \
-    }                                                                           \
-    if (up) qsel += (NH);                                                       \
-  }
-  int done_bits = 0;
-  if constexpr (LD >= 2) { COSMO_HALVE(LD / 2, 32); done_bits |= 32; }
-  if constexpr (LD >= 4) { COSMO_HALVE(LD / 4, 16); done_bits |= 16; }
-  if constexpr (LD >= 8) { COSMO_HALVE(LD / 8, 8); done_bits |= 8; }
-  if constexpr (LD >= 16) { COSMO_HALVE(LD / 16, 4); done_bits |= 4; }
-#undef COSMO_HALVE
-  T r = acc[0];
-  for (int off = 32; off > 0; off >>= 1)
-    if (!(done_bits & off)) r += __shfl_xor(r, off);
-  if ((lane & ~done_bits) == 0 && qsel < m) Y[row * LD + qsel] = r;
-}
-
 // FP64 product summed per output element over the full row in ascending column order,
 // one multiply then one add per entry (no contraction): the summation order of
 // SparseMatrixCSB::TimesMatrix for a symmetric matrix (include/tav/csbmatrix.cxx:623-663,
@@ -878,20 +807,6 @@ static bool launch_spmm_vec(const CsrDev &M, const T *vals, const T *X, T *Y, in
   const int lpe = ld / VPL;
   dim3 g((unsigned)((M.n + 3) / 4)), b(256);
   const int rm = g_spmm_remap;
-  static int variant = -1;
-  if (variant < 0) variant = getenv("COSMO_SPMM_VARIANT") ? atoi(getenv("COSMO_SPMM_VARIANT")) : 1;
-  if (variant == 2 && ld <= 16) {
-    switch (ld) {
-      case 2:
-        if (VPL > 2) break;
-        hipLaunchKernelGGL((k_spmm_lane<T, (VPL <= 2 ? 2 : 4)>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, rm);
-        return true;
-      case 4: hipLaunchKernelGGL((k_spmm_lane<T, 4>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, rm); return true;
-      case 8: hipLaunchKernelGGL((k_spmm_lane<T, 8>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, rm); return true;
-      case 16: hipLaunchKernelGGL((k_spmm_lane<T, 16>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, rm); return true;
-      default: break;
-    }
-  }
   switch (lpe) {
     case 1: hipLaunchKernelGGL((k_spmm_vec<T, 1>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
     case 2: hipLaunchKernelGGL((k_spmm_vec<T, 2>), g, b, 0, st, M.n, M.rowptr, M.cols, vals, X, Y, m, ld, rm); break;
@@ -945,6 +860,10 @@ int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int pr
   if (!env_done) {
     if (const char *e = getenv("COSMO_SPMM_REMAP")) g_spmm_remap = atoi(e);
     env_done = true;
+  }
+  if (op.fact && prec != COSMO_PREC_FP64_ORDERED && ldx == ldy) {
+    const int vpl = prec == COSMO_PREC_FP32 ? 4 : 2;
+    if (ldx % vpl == 0 && !(ldx & (ldx - 1))) return fact_apply(op, X, Y, m, ldx, prec, st);
   }
   if (op.is_virtual) {
     for (int v0 = 0; v0 < m; v0 += 16) {

@@ -25,11 +25,16 @@ struct Op {
   // staging for the host-pointer apply
   void *sx = nullptr, *sy = nullptr, *sx32 = nullptr, *sy32 = nullptr;
   size_t s_elems = 0;
+  void *fact = nullptr;        // product-form tables (COSMO_OP_FACTORED)
   double build_ms = 0, last_apply_ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
 int build_stored(Op &op, const System &s, int flags, hipStream_t st);
+int build_factored(Op &op, const System &s);
+void free_factored(Op &op);
+size_t factored_bytes(const Op &op);
+int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);
 int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int prec, hipStream_t st);
 int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);

@@ -23,6 +23,7 @@ static void free_op(Op *op) {
   hipFree(M.v32);
   hipFree(M.diag);
   free_tables(op->tab);
+  free_factored(*op);
   hipFree(op->sx);
   hipFree(op->sy);
   hipFree(op->sx32);
@@ -137,10 +138,11 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
   op->device = current_device();
   op->n = s->dim;
   op->flags = flags;
-  op->is_virtual = (flags & COSMO_OP_VIRTUAL) != 0;
+  op->is_virtual = (flags & (COSMO_OP_VIRTUAL | COSMO_OP_FACTORED)) != 0;
   hipStream_t st = lib_stream();
   auto t0 = std::chrono::steady_clock::now();
   int rc = upload_tables(*s, op->tab, st);
+  if (!rc && (flags & COSMO_OP_FACTORED)) rc = build_factored(*op, *s);
   if (!rc) {
     if (op->is_virtual) {
       op->csr.n = op->n;
@@ -239,7 +241,7 @@ int64_t cosmo_op_bytes(cosmo_op op) {
   const CsrDev &M = op->csr;
   int64_t b = (int64_t)op->tab.bytes + 8 * M.n;
   if (!op->is_virtual) b += 8 * (M.n + 1) + M.nnz * (4 + 8 + (M.v32 ? 4 : 0));
-  return b;
+  return b + (int64_t)factored_bytes(*op);
 }
 
 int cosmo_op_diagonal(cosmo_op op, double *d) {
@@ -264,27 +266,35 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
   hipSetDevice(op->device);
   hipStream_t st = lib_stream();
   const int64_t n = op->n;
-  int rc = ensure_staging(*op, (size_t)n * m);
+  // the product-form apply wants a power-of-two row pitch of at least 16 bytes
+  int ld = m;
+  if (op->fact) {
+    ld = 4;
+    while (ld < m) ld <<= 1;
+  }
+  int rc = ensure_staging(*op, (size_t)n * ld);
   if (rc) return rc;
   std::vector<double> h((size_t)n * m);
   for (int v = 0; v < m; ++v)
     for (int64_t r = 0; r < n; ++r) h[(size_t)r * m + v] = x[v][r];
-  HIPCHK(hipMemcpyAsync(op->sx, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(op->sx, sizeof(double) * ld, h.data(), sizeof(double) * m, sizeof(double) * m, n,
+                          hipMemcpyHostToDevice, st));
   if (precision == COSMO_PREC_FP32) {
-    rc = cast_pack(n, m, (const double *)op->sx, m, (float *)op->sx32, m, st);
+    rc = cast_pack(n, m, (const double *)op->sx, ld, (float *)op->sx32, ld, st);
     if (rc) return rc;
     HIPCHK(hipEventRecord(op->e0, st));
-    rc = op_apply_dev(*op, op->sx32, op->sy32, m, m, m, precision, st);
+    rc = op_apply_dev(*op, op->sx32, op->sy32, m, ld, ld, precision, st);
     HIPCHK(hipEventRecord(op->e1, st));
     if (rc) return rc;
-    rc = cast_unpack(n, m, (const float *)op->sy32, m, (double *)op->sy, m, st);
+    rc = cast_unpack(n, m, (const float *)op->sy32, ld, (double *)op->sy, ld, st);
   } else {
     HIPCHK(hipEventRecord(op->e0, st));
-    rc = op_apply_dev(*op, op->sx, op->sy, m, m, m, precision, st);
+    rc = op_apply_dev(*op, op->sx, op->sy, m, ld, ld, precision, st);
     HIPCHK(hipEventRecord(op->e1, st));
   }
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(h.data(), op->sy, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpy2DAsync(h.data(), sizeof(double) * m, op->sy, sizeof(double) * ld, sizeof(double) * m, n,
+                          hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float ms = 0;
   hipEventElapsedTime(&ms, op->e0, op->e1);

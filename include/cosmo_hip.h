@@ -97,6 +97,9 @@ int cosmo_system_check_mbs(cosmo_system s, const char *path);
 #define COSMO_OP_KEEP_SMALL 0x2  /* keep |h| < 1e-14 (the .HH content); default drops */
                                  /* them as the CSB builder does (PPMatrix.cxx:621)   */
 #define COSMO_OP_NO_FP32 0x4     /* stored: skip the FP32 value copy                   */
+#define COSMO_OP_FACTORED 0x8    /* virtual, in proton-neutron product form: H.X from  */
+                                 /* per-species tables (no H in HBM); FP32 and FP64    */
+                                 /* applies use it, FP64_ORDERED uses the row generator */
 
 #define COSMO_PREC_FP64 0        /* FP64 values, FP64 vectors, FP64 accumulation       */
 #define COSMO_PREC_FP32 1        /* FP32 values, FP32 vectors, FP32 accumulation       */

@@ -75,7 +75,7 @@ def test_apply_device_torch(gpu, golden):
     n = op.n
     X = torch.randn(n, 16, dtype=torch.float64, device="cuda")
     Y = torch.empty_like(X)
-    op.apply_device(X, Y, gpu.PREC_FP64)
+    op.apply_device(X, Y, gpu.PREC_FP64, stream=torch.cuda.current_stream())
     torch.cuda.synchronize()
     ref = op.apply(X.cpu().numpy(), gpu.PREC_FP64)
     assert np.max(np.abs(Y.cpu().numpy() - ref)) < 1e-12 * np.abs(ref).max()
