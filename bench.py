@@ -6,12 +6,24 @@ isospin interaction) with k = 16 FP32 vectors, H built on the GPU beforehand and
 resident in HBM.  value = algorithmic H.X bandwidth in GB/s:
     B(k) = nnz_upper * (4 + 2 + 2) + n * k * (4 + 4)      (FP32 value, two u16 CSB indices,
                                                          X read once, Y written once)
-Also reported: the roofline of that kernel, a CPU baseline (the CSB product restated in
-C with OpenMP, FP64 like the reference, on a row sample of the same matrix), and a
-10-eigenpair block-16 Davidson solve in all-FP64 and in mixed precision with the
-per-eigenvalue difference (MeV) and solve rates.
+Reported beside it:
+  * hv_virtual: the same product with the proton-neutron product-form operator (no H
+    stored), FP32 and FP64, with its B(k)-equivalent rate and the difference to the
+    stored product;
+  * roofline of the stored kernel (HIP events on its stream; HBM traffic from rocprofv3
+    counters when --traffic-json names their summary);
+  * solve: 10 lowest eigenpairs, block 16, all-FP64 and mixed precision, on both
+    operators, with the per-eigenvalue difference (MeV) and solve rates;
+  * batch: 256 independent sd-shell solves of uneven size (Nv, Zv ~ U{1..6}, 2Jz from
+    the allowed range, own seed each), mixed precision, 5 pairs each;
+  * serve: a Poisson request stream (sd 70 % / fp 30 %, fp up to 4+4, k ~ U{1..10},
+    tolerance log-uniform in [1e-6, 1e-4]) through the admission-controlled server,
+    p50/p99 latency, solves/s and each result's difference to an FP64 solve alone;
+  * cpu_baseline: the CSB product restated in C with OpenMP (FP64, as the reference) on
+    a row sample of the same matrix, on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-batch]
+                       [--no-serve] [--no-cpu] [--calibrate-copy]
 """
 import argparse
 import json
@@ -27,19 +39,44 @@ METRIC = "H·V GB/s (% HBM peak) + solves/s, p99 latency; mixed vs FP64 at ≤1 
 HBM_PEAK_GBS = 8000.0
 
 
+def timed_applies(op, X, Y, prec, stream, warmup, steps):
+    """Wall time per apply and per-apply event times, all on `stream`."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        op.apply_device(X, Y, prec, stream)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for (a, b) in evs:
+        a.record(stream)
+        op.apply_device(X, Y, prec, stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, np.array([a.elapsed_time(b) for (a, b) in evs])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--shell", default="fp")
     ap.add_argument("--nv", type=int, default=4)
     ap.add_argument("--zv", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-virtual", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline row sample (0: n/8)")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_spmm_vec_fp4p4n_k16_traffic.json"),
+                    help="HBM bytes per launch of the stored kernel from rocprofv3 counters")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 4 GiB device copy three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the H.V path is measured on one MI355X); "
@@ -52,6 +89,15 @@ def main():
     from cosmo_amd import synth
 
     torch.cuda.init()
+    if args.calibrate_copy:
+        x = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "launches": 3}))
+        return
+
     tmp = tempfile.mkdtemp(prefix="cosmo_bench_")
     name = "%s%dp%dn" % (args.shell, args.zv, args.nv)
     xml = synth.write_system(tmp, args.shell, name, args.nv, args.zv, 0, "+", args.seed)
@@ -73,42 +119,30 @@ def main():
     X = torch.randn(n, k, device="cuda", dtype=torch.float32, generator=g)
     Y = torch.empty_like(X)
     stream = torch.cuda.Stream()
-    for _ in range(args.warmup):
-        op.apply_device(X, Y, ca.PREC_FP32, stream)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for (a, b) in evs:
-        a.record(stream)
-        op.apply_device(X, Y, ca.PREC_FP32, stream)
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = np.array([a.elapsed_time(b) for (a, b) in evs])
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step, kms = timed_applies(op, X, Y, ca.PREC_FP32, stream, args.warmup, args.steps)
     bytes_alg = nnz_u * (4 + 2 + 2) + n * k * (4 + 4)
     flops = 2.0 * k * (2 * nnz_u - n)
     gbs = bytes_alg / (ms_per_step * 1e-3) / 1e9
     kavg = float(kms.mean())
     achieved = bytes_alg / (kavg * 1e-3) / 1e9
-    print("[bench] H.V k=%d: %.3f ms/step wall, kernel avg %.3f ms -> %.1f GB/s algorithmic" %
+    print("[bench] stored H.V k=%d: %.3f ms/step wall, kernel avg %.3f ms -> %.1f GB/s algorithmic" %
           (k, ms_per_step, kavg, achieved), file=sys.stderr, flush=True)
 
-    # correctness spot check of the timed product: a few rows against FP64 host rows
+    # correctness spot check of the timed product: rows of H (columns, H symmetric) times X
     rows = np.random.default_rng(0).choice(n, 8, replace=False)
     Xh = X.double().cpu().numpy()
     Yh = Y.double().cpu().numpy()
     maxrel = 0.0
     for r in rows:
-        rp, c, v = op.upper_rows(int(r), int(r) + 1)
-        # full row r = lower part (from other rows) + upper part; check the upper part via
-        # the symmetric product of a unit vector instead: y_r = sum_c H[r,c] x_c
         e = np.zeros((n, 1))
         e[r, 0] = 1.0
-        hr = op.apply(e, ca.PREC_FP64)[:, 0]  # column r of H = row r
+        hr = op.apply(e, ca.PREC_FP64)[:, 0]
         ref = hr @ Xh
         maxrel = max(maxrel, float(np.max(np.abs(Yh[r] - ref)) / max(1e-30, np.max(np.abs(ref)))))
+
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        traffic = json.load(open(args.traffic_json)).get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -121,39 +155,188 @@ def main():
         "hv": {"frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "gflops": round(flops / (ms_per_step * 1e-3) / 1e9, 1),
                "kernel_ms_p50": round(float(np.percentile(kms, 50)), 4),
                "kernel_ms_p99": round(float(np.percentile(kms, 99)), 4), "spot_check_max_rel_err": maxrel,
-               "build_s": round(t_build, 3), "setup_s": round(t_setup, 3)},
+               "build_s": round(t_build, 3), "setup_s": round(t_setup, 3), "device_bytes": op.bytes()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmm<float,float,float,16>", "bytes_per_launch": bytes_alg},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_spmm_vec<float,4>", "bytes_per_launch": bytes_alg},
     }
 
+    print("[bench] spot check of %d rows: max rel err %.2e" % (len(rows), maxrel), file=sys.stderr, flush=True)
+    fop = None
+    if not args.no_virtual:
+        fop, out["hv_virtual"] = virtual_section(ca, sysm, op, X, Y, stream, args, bytes_alg, ms_per_step)
     if not args.no_solve:
-        out["solve"] = solve_section(ca, op, n)
+        out["solve"] = solve_section(ca, fop if fop is not None else op, op, n)
+    del X, Y
+    fop = None  # frees the product-form operator
+    if not args.no_batch:
+        out["batch"] = batch_section(ca, synth)
+    if not args.no_serve:
+        out["serve"] = serve_section(ca, synth)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_section(ca, op, n, k, args.cpu_rows)
     print(json.dumps(out), flush=True)
 
 
-def solve_section(ca, op, n):
+def virtual_section(ca, sysm, op, X, Y, stream, args, bytes_alg, ms_stored):
+    """Product-form operator: build, FP32 and FP64 applies of the same X, agreement."""
+    import numpy as np
+    import torch
+    t0 = time.perf_counter()
+    fop = ca.Operator.build(sysm, ca.OP_FACTORED)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    Yf = torch.empty_like(X)
+    ms32, k32 = timed_applies(fop, X, Yf, ca.PREC_FP32, stream, args.warmup, args.steps)
+    rel32 = float((Yf - Y).abs().max() / Y.abs().max())
+    X64 = X.double()
+    Y64 = torch.empty_like(X64)
+    ms64, k64 = timed_applies(fop, X64, Y64, ca.PREC_FP64, stream, 2, max(5, args.steps // 4))
+    Ys64 = torch.empty_like(X64)
+    op.apply_device(X64, Ys64, ca.PREC_FP64, stream)
+    torch.cuda.synchronize()
+    rel64 = float((Y64 - Ys64).abs().max() / Ys64.abs().max())
+    print("[bench] product-form H.V k=%d: fp32 %.3f ms, fp64 %.3f ms (stored fp32 %.3f ms); rel diff fp32 %.1e "
+          "fp64 %.1e" % (X.shape[1], ms32, ms64, ms_stored, rel32, rel64), file=sys.stderr, flush=True)
+    del X64, Y64, Ys64, Yf
+    return fop, {
+        "operator": "proton-neutron product form (no H stored)", "build_s": round(t_build, 3),
+        "device_bytes": fop.bytes(),
+        "fp32": {"ms_per_step": round(ms32, 4), "event_ms_p50": round(float(np.percentile(k32, 50)), 4),
+                 "equiv_GBs": round(bytes_alg / (ms32 * 1e-3) / 1e9, 1), "speedup_vs_stored": round(ms_stored / ms32, 3),
+                 "max_rel_diff_vs_stored_fp32": rel32},
+        "fp64": {"ms_per_step": round(ms64, 4), "event_ms_p50": round(float(np.percentile(k64, 50)), 4),
+                 "max_rel_diff_vs_stored_fp64": rel64},
+        "note": "equiv_GBs is B(k) of the stored product divided by this apply's time",
+    }
+
+
+def solve_section(ca, fop, op, n):
+    """10 lowest pairs, block 16, all-FP64 and mixed, on the product-form operator (the
+    faster one) and on the stored one."""
     import numpy as np
     res = {}
-    for label, prec in (("fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
-        t0 = time.perf_counter()
-        ev, de, rs, _, st = ca.davidson(op, 10, mblock=16, lim=60, want_vectors=False, critr=1e-6, precision=prec)
-        dt = time.perf_counter() - t0
-        res[label] = dict(seconds=round(dt, 3), evals=[round(float(x), 8) for x in ev],
-                          max_resid=float(np.max(rs)), iterations=st["iterations"], matvecs=st["matvecs"],
-                          matvecs_fp64=st["matvecs_fp64"], ms_apply=round(st["ms_apply"], 1),
-                          ms_ortho=round(st["ms_ortho"], 1), ms_rr=round(st["ms_rr"], 1), converged=st["converged"])
-        print("[bench] solve %s: %.2fs iters %d resid %.2e" % (label, dt, st["iterations"], np.max(rs)),
-              file=sys.stderr, flush=True)
-    d = np.abs(np.array(res["mixed"]["evals"]) - np.array(res["fp64"]["evals"]))
-    res["max_abs_dE_MeV"] = float(d.max())
-    res["within_1keV"] = bool(d.max() <= 1e-3)
-    res["mixed_speedup"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
-    res["solves_per_s_mixed"] = round(1.0 / res["mixed"]["seconds"], 4)
-    res["eigenpairs_per_s_mixed"] = round(10.0 / res["mixed"]["seconds"], 3)
+    for oname, o in (("virtual", fop), ("stored", op)):
+        if o is None:
+            continue
+        r = {}
+        for label, prec in (("fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
+            t0 = time.perf_counter()
+            ev, de, rs, _, st = ca.davidson(o, 10, mblock=16, lim=60, want_vectors=False, critr=1e-6, precision=prec)
+            dt = time.perf_counter() - t0
+            r[label] = dict(seconds=round(dt, 3), evals=[round(float(x), 8) for x in ev],
+                            max_resid=float(np.max(rs)), iterations=st["iterations"], matvecs=st["matvecs"],
+                            matvecs_fp64=st["matvecs_fp64"], ms_apply=round(st["ms_apply"], 1),
+                            ms_ortho=round(st["ms_ortho"], 1), ms_rr=round(st["ms_rr"], 1), converged=st["converged"])
+            print("[bench] solve %s %s: %.2fs iters %d resid %.2e" % (oname, label, dt, st["iterations"], np.max(rs)),
+                  file=sys.stderr, flush=True)
+        d = np.abs(np.array(r["mixed"]["evals"]) - np.array(r["fp64"]["evals"]))
+        r["abs_dE_MeV"] = [float(x) for x in d]
+        r["max_abs_dE_MeV"] = float(d.max())
+        r["within_1keV"] = bool(d.max() <= 1e-3)
+        r["mixed_speedup"] = round(r["fp64"]["seconds"] / r["mixed"]["seconds"], 3)
+        r["solves_per_s_mixed"] = round(1.0 / r["mixed"]["seconds"], 4)
+        r["eigenpairs_per_s_mixed"] = round(10.0 / r["mixed"]["seconds"], 3)
+        res[oname] = r
     return res
+
+
+def random_systems(ca, synth, count, seed, shells=("sd",), p_sd=1.0, max_sd=6, max_fp=4):
+    """Seeded random requests: shell, Nv, Zv, 2Jz from the allowed range, own interaction."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    d = tempfile.mkdtemp(prefix="cosmo_req_")
+    out = []
+    while len(out) < count:
+        shell = "sd" if rng.random() < p_sd else "fp"
+        cap = max_sd if shell == "sd" else max_fp
+        nv, zv = int(rng.integers(1, cap + 1)), int(rng.integers(1, cap + 1))
+        A = nv + zv
+        jmax = 2 * A  # generous; empty M-spaces are redrawn
+        twojz = int(rng.choice([j for j in range(-jmax, jmax + 1) if (j - A) % 2 == 0]))
+        par = "+" if shell == "sd" else ("-" if A % 2 else "+")
+        xml = synth.write_system(d, shell, "r%d" % len(out), nv, zv, twojz, par, int(rng.integers(1 << 30)))
+        s = ca.System.from_xml(xml)
+        if s.dim < 1:
+            continue
+        out.append((shell, nv, zv, twojz, s))
+    return out, rng
+
+
+def batch_section(ca, synth):
+    """256 independent sd-shell solves of uneven size in one batch, mixed precision."""
+    import numpy as np
+    t0 = time.perf_counter()
+    reqs, _ = random_systems(ca, synth, 256, 17)
+    systems = [r[4] for r in reqs]
+    nume = [min(5, s.dim) for s in systems]
+    t_setup = time.perf_counter() - t0
+    print("[bench] batch: %d systems set up in %.2fs, building operators" % (len(systems), t_setup),
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = ca.Batch(systems)
+    t_build = time.perf_counter() - t0
+    print("[bench] batch: operators built in %.2fs, solving" % t_build, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ev, rs, st = b.solve(nume, critr=1e-6, precision=ca.PREC_FP32)
+    dt = time.perf_counter() - t0
+    b.close()
+    dims = np.array([s.dim for s in systems])
+    maxres = float(max(np.max(r) for r in rs))
+    print("[bench] batch: 256 solves (n %d..%d) built %.2fs solved %.2fs" % (dims.min(), dims.max(), t_build, dt),
+          file=sys.stderr, flush=True)
+    return {"solves": len(systems), "dims_min": int(dims.min()), "dims_median": int(np.median(dims)),
+            "dims_max": int(dims.max()), "eigenpairs": int(sum(nume)), "setup_s": round(t_setup, 3),
+            "build_s": round(t_build, 3), "solve_s": round(dt, 3), "solves_per_s": round(len(systems) / dt, 2),
+            "eigenpairs_per_s": round(sum(nume) / dt, 2), "max_resid": maxres, "converged": st["converged"],
+            "precision": "mixed (FP32 H.V, FP64 refinement)"}
+
+
+def serve_section(ca, synth, count=48, rate=12.0):
+    """Poisson arrivals through the server; latency, throughput and FP64 agreement."""
+    import numpy as np
+    reqs, rng = random_systems(ca, synth, count, 23, p_sd=0.7, max_sd=6, max_fp=4)
+    ks = [int(min(rng.integers(1, 11), r[4].dim)) for r in reqs]
+    tols = [float(10 ** rng.uniform(-6, -4)) for _ in reqs]
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    print("[bench] serve: %d requests at %.1f/s" % (len(reqs), rate), file=sys.stderr, flush=True)
+    srv = ca.Server(hbm_budget=96 << 30, max_active=8, precision=ca.PREC_FP32)
+    ids = []
+    t0 = time.perf_counter()
+    for (r, k, tol, gap) in zip(reqs, ks, tols, gaps):
+        time.sleep(float(gap))
+        ids.append(srv.submit(r[4], k, tol))
+    res = srv.drain()
+    wall = time.perf_counter() - t0
+    srv.close()
+    byid = {x["id"]: x for x in res}
+    lat = np.array([byid[i]["t_done"] - byid[i]["t_submit"] for i in ids])
+    queue = np.array([byid[i]["t_admit"] - byid[i]["t_submit"] for i in ids])
+    ok = all(byid[i]["status"] == 0 for i in ids)
+    # accuracy of each result against an all-FP64 solve of the same request alone
+    dE = []
+    for i, (r, k, tol) in zip(ids, zip(reqs, ks, tols)):
+        s = r[4]
+        if s.dim > 100000:
+            continue
+        o = ca.Operator.build(s)
+        e64 = ca.davidson(o, k, want_vectors=False, critr=tol, lim=k + max(20, k), maxiter=4000)[0]
+        dE.append(float(np.max(np.abs(byid[i]["evals"] - e64))))
+        del o
+        if len(dE) % 8 == 0:
+            print("[bench] serve: %d results checked against FP64" % len(dE), file=sys.stderr, flush=True)
+    print("[bench] serve: %d requests in %.2fs, p50 %.3fs p99 %.3fs" % (len(ids), wall, np.percentile(lat, 50),
+                                                                        np.percentile(lat, 99)),
+          file=sys.stderr, flush=True)
+    shells = [r[0] for r in reqs]
+    return {"requests": len(ids), "rate_per_s": rate, "sd_fraction": round(shells.count("sd") / len(shells), 3),
+            "dims_max": int(max(r[4].dim for r in reqs)), "wall_s": round(wall, 3),
+            "solves_per_s": round(len(ids) / wall, 3), "eigenpairs_per_s": round(sum(ks) / wall, 3),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "queue_p50_s": round(float(np.percentile(queue, 50)), 4), "all_ok": ok,
+            "checked_vs_fp64": len(dE), "max_abs_dE_vs_fp64_MeV": max(dE) if dE else None,
+            "within_1keV": bool(dE and max(dE) <= 1e-3), "precision": "mixed"}
 
 
 def cpu_section(ca, op, n, k, rows):

@@ -274,11 +274,11 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
   }
   int rc = ensure_staging(*op, (size_t)n * ld);
   if (rc) return rc;
-  std::vector<double> h((size_t)n * m);
+  // rows padded to ld on the host, so each direction is one contiguous copy
+  std::vector<double> h((size_t)n * ld, 0.0);
   for (int v = 0; v < m; ++v)
-    for (int64_t r = 0; r < n; ++r) h[(size_t)r * m + v] = x[v][r];
-  HIPCHK(hipMemcpy2DAsync(op->sx, sizeof(double) * ld, h.data(), sizeof(double) * m, sizeof(double) * m, n,
-                          hipMemcpyHostToDevice, st));
+    for (int64_t r = 0; r < n; ++r) h[(size_t)r * ld + v] = x[v][r];
+  HIPCHK(hipMemcpyAsync(op->sx, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
   if (precision == COSMO_PREC_FP32) {
     rc = cast_pack(n, m, (const double *)op->sx, ld, (float *)op->sx32, ld, st);
     if (rc) return rc;
@@ -293,14 +293,13 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
     HIPCHK(hipEventRecord(op->e1, st));
   }
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(h.data(), sizeof(double) * m, op->sy, sizeof(double) * ld, sizeof(double) * m, n,
-                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h.data(), op->sy, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float ms = 0;
   hipEventElapsedTime(&ms, op->e0, op->e1);
   op->last_apply_ms = ms;
   for (int v = 0; v < m; ++v)
-    for (int64_t r = 0; r < n; ++r) y[v][r] = h[(size_t)r * m + v];
+    for (int64_t r = 0; r < n; ++r) y[v][r] = h[(size_t)r * ld + v];
   return COSMO_OK;
 }
 
