@@ -10,7 +10,7 @@
 // Orthogonalisation is blocked: the new vectors are projected against the existing
 // basis with two classical Gram-Schmidt sweeps (B^T V and V - B W as two tall-skinny
 // passes) and then orthonormalised among themselves by a pivot-dropping Cholesky QR,
-// repeated once.  This spans the same space as the reference's per-vector repeated CGS
+// repeated once in the FP64 phase (not while H.X runs in FP32).  This spans the same space as the reference's per-vector repeated CGS
 // (Orthogonalize_Basis :313-390) in a few passes over B instead of one per new vector.
 //
 // Mixed precision (COSMO_PREC_FP32): H.X runs on the FP32 values with FP32 vectors until
@@ -73,11 +73,23 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const double *__restric
   }
 }
 
-__global__ void k_reduce_partial(int G, int pq, const double *__restrict__ partial, double *__restrict__ out) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of the G Gram partials in two fixed-order stages (deterministic): stage one sums
+// GRAM_SEG contiguous segments of blocks in parallel, stage two the segment sums.
+#define GRAM_SEG 32
+__global__ void k_reduce_partial(int G, int pq, const double *__restrict__ partial, double *__restrict__ seg) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= pq) return;
+  const int per = (G + GRAM_SEG - 1) / GRAM_SEG;
+  const int g0 = blockIdx.y * per, g1 = min(G, g0 + per);
+  double s = 0.0;
+  for (int g = g0; g < g1; ++g) s += partial[(int64_t)g * pq + o];
+  seg[(int64_t)blockIdx.y * pq + o] = s;
+}
+__global__ void k_reduce_segments(int pq, const double *__restrict__ seg, double *__restrict__ out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= pq) return;
   double s = 0.0;
-  for (int g = 0; g < G; ++g) s += partial[(int64_t)g * pq + o];
+  for (int k = 0; k < GRAM_SEG; ++k) s += seg[(int64_t)k * pq + o];
   out[o] = s;
 }
 
@@ -183,44 +195,46 @@ struct Dav {
   int lim = 0, nume = 0, mblock = 0;
   double *B = nullptr, *AB = nullptr, *R = nullptr, *T = nullptr, *Xd = nullptr, *Yd = nullptr;
   float *Xf = nullptr, *Yf = nullptr;
-  double *partial = nullptr, *small = nullptr, *small2 = nullptr, *lam = nullptr;
+  double *partial = nullptr, *segs = nullptr, *small = nullptr, *small2 = nullptr, *lam = nullptr;
   int G = 0;
   double ms_apply = 0, ms_ortho = 0, ms_rr = 0;
   int matvecs = 0, matvecs64 = 0;
   ~Dav() {
-    hipFree(B);
-    hipFree(AB);
-    hipFree(R);
-    hipFree(T);
-    hipFree(Xd);
-    hipFree(Yd);
-    hipFree(Xf);
-    hipFree(Yf);
-    hipFree(partial);
-    hipFree(small);
-    hipFree(small2);
-    hipFree(lam);
+    dfree(B);
+    dfree(AB);
+    dfree(R);
+    dfree(T);
+    dfree(Xd);
+    dfree(Yd);
+    dfree(Xf);
+    dfree(Yf);
+    dfree(partial);
+    dfree(segs);
+    dfree(small);
+    dfree(small2);
+    dfree(lam);
   }
   double *col(double *base, int j) { return base + (int64_t)j * n; }
 };
 
 static int dav_alloc(Dav &D) {
   const int64_t n = D.n;
-  HIPCHK(hipMalloc(&D.B, sizeof(double) * n * D.lim));
-  HIPCHK(hipMalloc(&D.AB, sizeof(double) * n * D.lim));
-  HIPCHK(hipMalloc(&D.R, sizeof(double) * n * std::max(D.nume, D.mblock)));
-  HIPCHK(hipMalloc(&D.T, sizeof(double) * n * std::max(D.lim, 16)));
+  HIPCHK(dmalloc(&D.B, sizeof(double) * n * D.lim));
+  HIPCHK(dmalloc(&D.AB, sizeof(double) * n * D.lim));
+  HIPCHK(dmalloc(&D.R, sizeof(double) * n * std::max(D.nume, D.mblock)));
+  HIPCHK(dmalloc(&D.T, sizeof(double) * n * std::max(D.lim, 16)));
   int ldmax = 4;
   while (ldmax < D.mblock) ldmax <<= 1;
-  HIPCHK(hipMalloc(&D.Xd, sizeof(double) * n * ldmax));
-  HIPCHK(hipMalloc(&D.Yd, sizeof(double) * n * ldmax));
-  HIPCHK(hipMalloc(&D.Xf, sizeof(float) * n * ldmax));
-  HIPCHK(hipMalloc(&D.Yf, sizeof(float) * n * ldmax));
+  HIPCHK(dmalloc(&D.Xd, sizeof(double) * n * ldmax));
+  HIPCHK(dmalloc(&D.Yd, sizeof(double) * n * ldmax));
+  HIPCHK(dmalloc(&D.Xf, sizeof(float) * n * ldmax));
+  HIPCHK(dmalloc(&D.Yf, sizeof(float) * n * ldmax));
   D.G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + GRAM_ROWS - 1) / GRAM_ROWS));
-  HIPCHK(hipMalloc(&D.partial, sizeof(double) * (size_t)D.G * 1024));
-  HIPCHK(hipMalloc(&D.small, sizeof(double) * 8192));
-  HIPCHK(hipMalloc(&D.small2, sizeof(double) * 8192));
-  HIPCHK(hipMalloc(&D.lam, sizeof(double) * 256));
+  HIPCHK(dmalloc(&D.partial, sizeof(double) * (size_t)D.G * 1024));
+  HIPCHK(dmalloc(&D.segs, sizeof(double) * (size_t)GRAM_SEG * 1024));
+  HIPCHK(dmalloc(&D.small, sizeof(double) * 8192));
+  HIPCHK(dmalloc(&D.small2, sizeof(double) * 8192));
+  HIPCHK(dmalloc(&D.lam, sizeof(double) * 256));
   HIPCHK(hipMemsetAsync(D.B, 0, sizeof(double) * n * D.lim, D.st));
   HIPCHK(hipMemsetAsync(D.AB, 0, sizeof(double) * n * D.lim, D.st));
   return COSMO_OK;
@@ -237,8 +251,9 @@ static int gram(Dav &D, const double *A, int p, const double *Bm, int q, std::ve
     int pp = std::min(pc, p - a0);
     hipLaunchKernelGGL(k_gram, dim3(D.G), dim3(256), 0, D.st, D.n, A + (int64_t)a0 * D.n, D.n, pp, Bm, D.n, q,
                        D.partial);
-    hipLaunchKernelGGL(k_reduce_partial, dim3((pp * q + 255) / 256), dim3(256), 0, D.st, D.G, pp * q, D.partial,
-                       D.small);
+    hipLaunchKernelGGL(k_reduce_partial, dim3((pp * q + 255) / 256, GRAM_SEG), dim3(256), 0, D.st, D.G, pp * q,
+                       D.partial, D.segs);
+    hipLaunchKernelGGL(k_reduce_segments, dim3((pp * q + 255) / 256), dim3(256), 0, D.st, pp * q, D.segs, D.small);
     HIPCHK(hipGetLastError());
     tmp.resize((size_t)pp * q);
     HIPCHK(hipMemcpyAsync(tmp.data(), D.small, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, D.st));
@@ -379,10 +394,13 @@ static void sym_eig(int k, const std::vector<double> &A0, std::vector<double> &w
   V.swap(Vr);
 }
 
-// Orthonormalise V (q columns at Vp) against B[:, 0..p) and among themselves.
+// Orthonormalise V (q columns at Vp) against B[:, 0..p) and among themselves, in
+// `rounds` rounds of Gram-Schmidt passes plus Cholesky QR (two for FP64 accuracy; one
+// while H.V itself is FP32, whose vectors are only good to single precision).
 // Returns the number of kept vectors (compacted to the front); restart=true when the
 // overlap with the basis does not decrease (the reference's restart condition).
-static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &restart, int &kept) {
+static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &restart, int &kept,
+                          int rounds = 2) {
   auto t0 = std::chrono::steady_clock::now();
   restart = false;
   kept = q;
@@ -404,7 +422,7 @@ static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(D.st));
   }
-  for (int round = 0; round < 2 && kept > 0; ++round) {
+  for (int round = 0; round < rounds && kept > 0; ++round) {
     double dprev = 1e7;
     for (int pass = 0; pass < 4; ++pass) {
       if (!p) break;
@@ -574,6 +592,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   Dav D;
   D.op = op;
   D.st = lib_stream();
+  note_stream(*op, D.st);
   D.n = n;
   D.lim = p.lim;
   D.nume = p.nume;
@@ -772,7 +791,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
       }
       bool restart = false;
       int kept = nadded;
-      rc = orthonormalise(D, kpass, D.col(D.B, kpass), nadded, p.ortho, restart, kept);
+      rc = orthonormalise(D, kpass, D.col(D.B, kpass), nadded, p.ortho, restart, kept, fp64_phase ? 2 : 1);
       if (rc) return rc;
       nncv = kept;
       if (!restart) first = true;

@@ -57,4 +57,17 @@ int upload_tables(const System &s, DevTablesOwner &o, hipStream_t st);
 void free_tables(DevTablesOwner &o);
 
 hipStream_t lib_stream();
+
+// Library device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the calling
+// thread's library stream, from the device's pool with no release threshold): a free
+// does not synchronise the device, so solves running concurrently from several host
+// threads (batches, the server) do not stall one another, and memory freed by one solve
+// is reused by the next.  Memory used on a foreign stream is freed only after the device
+// has been synchronised (see cosmo_op_destroy).
+hipError_t dmalloc_void(void **p, size_t bytes, hipStream_t st);
+void dfree(void *p);
+template <typename T>
+inline hipError_t dmalloc(T **p, size_t bytes, hipStream_t st = nullptr) {
+  return dmalloc_void(reinterpret_cast<void **>(p), bytes, st);
+}
 }  // namespace cosmo

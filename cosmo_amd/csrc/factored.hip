@@ -67,6 +67,8 @@ struct FactArgs {
   const int32_t *nsector;       // [nblk]
   const uint32_t *nmask;        // [nblk]
   const int64_t *poff;          // [nblk] first row in the class-contiguous layout
+  const int64_t *qoff;          // [nblk] row of (proton 0, this block) in the proton-major layout
+  const int32_t *qnb;           // [nblk] blocks of its class (row stride of the proton-major layout)
   const int32_t *step_start;    // [nblk+1]
   const int32_t *step_tb;       // target block of each step
   const uint32_t *step_code;
@@ -107,7 +109,7 @@ __device__ __forceinline__ int xcd_block(int nblk) {
 template <typename T, int LPE, int RMAX>
 __global__ __launch_bounds__(FACT_THREADS) void k_fact(FactArgs A, const T *__restrict__ W,
                                                        const T *__restrict__ X, const T *__restrict__ Yp,
-                                                       T *__restrict__ Y, int ld, int m) {
+                                                       const T *__restrict__ Yq, T *__restrict__ Y, int ld, int m) {
   constexpr int VPL = Piece16<T>::N;
   // native vectors (not HIP's union vector types, which keep a register array in scratch)
   typedef T NV __attribute__((ext_vector_type(VPL)));
@@ -288,13 +290,14 @@ __global__ __launch_bounds__(FACT_THREADS) void k_fact(FactArgs A, const T *__re
     }
   }
   // epilogue: add the single-species products and store the columns below m
-  const int64_t pr0 = A.poff[blk];
+  const int64_t pr0 = A.poff[blk], qr0 = A.qoff[blk], qs = A.qnb[blk];
 #pragma unroll
   for (int j = 0; j < RMAX; ++j) {
     const int r = g + j * G;
     if (r < S) {
       const int c = col0 + sub * VPL;
-      const NV yn = *reinterpret_cast<const NV *>(Yp + (pr0 + r) * ld + c);
+      const NV yn = *reinterpret_cast<const NV *>(Yp + (pr0 + r) * ld + c) +
+                    *reinterpret_cast<const NV *>(Yq + (qr0 + r * qs) * ld + c);
       T *y = Y + (r0 + r) * ld + c;
       if (c + VPL <= m) {
         NV o;
@@ -310,17 +313,25 @@ __global__ __launch_bounds__(FACT_THREADS) void k_fact(FactArgs A, const T *__re
   }
 }
 
-// class-contiguous copy of X: block b's rows move to row poff[b]
+// class-contiguous copies of X: neutron-major (block b's rows at row poff[b], for the Hn
+// product) and proton-major (row pi of block b at row qoff[b] + pi * qnb[b], for Hp)
 template <typename T>
 __global__ __launch_bounds__(256) void k_class_gather(int nblk, const int64_t *__restrict__ noff,
-                                                      const int64_t *__restrict__ poff, const T *__restrict__ X,
-                                                      T *__restrict__ Xp, int ld) {
+                                                      const int64_t *__restrict__ poff,
+                                                      const int64_t *__restrict__ qoff, const int32_t *__restrict__ qnb,
+                                                      const T *__restrict__ X, T *__restrict__ Xp, T *__restrict__ Xq,
+                                                      int ld) {
   const int b = blockIdx.x;
   if (b >= nblk) return;
   const int64_t len = (noff[b + 1] - noff[b]) * ld;
   const T *src = X + noff[b] * ld;
   T *dst = Xp + poff[b] * ld;
-  for (int64_t i = threadIdx.x; i < len; i += 256) dst[i] = src[i];
+  const int64_t q0 = qoff[b], qs = qnb[b];
+  for (int64_t i = threadIdx.x; i < len; i += 256) {
+    const T x = src[i];
+    dst[i] = x;
+    Xq[(q0 + (i / ld) * qs) * ld + i % ld] = x;
+  }
 }
 
 }  // namespace
@@ -341,25 +352,29 @@ struct FactOp {
     const float *H32 = nullptr, *P32 = nullptr;
   };
   std::vector<Cls> cls;
-  void *Xp = nullptr, *Yp = nullptr;
+  void *Xp = nullptr, *Yp = nullptr, *Xq = nullptr, *Yq = nullptr;
   size_t p_bytes = 0;
   rocblas_handle blas = nullptr;
   int64_t n = 0;
   size_t bytes = 0;
   ~FactOp() {
-    for (void *p : owned) hipFree(p);
-    hipFree(Xp);
-    hipFree(Yp);
+    for (void *p : owned) dfree(p);
+    dfree(Xp);
+    dfree(Yp);
+    dfree(Xq);
+    dfree(Yq);
     if (blas) rocblas_destroy_handle(blas);
   }
   template <typename T>
   int put(const std::vector<T> &h, const T **out) {
     void *d = nullptr;
     const size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
-    if (hipMalloc(&d, b) != hipSuccess) return fail(COSMO_ENOMEM, "factorised operator: table allocation failed");
+    if (dmalloc(&d, b) != hipSuccess) return fail(COSMO_ENOMEM, "factorised operator: table allocation failed");
     owned.push_back(d);
     bytes += b;
-    if (!h.empty() && hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    if (!h.empty() && (hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, lib_stream()) !=
+                           hipSuccess ||
+                       hipStreamSynchronize(lib_stream()) != hipSuccess))
       return fail(COSMO_EHIP, "factorised operator: table upload failed");
     *out = (const T *)d;
     return COSMO_OK;
@@ -576,7 +591,8 @@ int build_factored(Op &op, const System &s) {
   for (auto &kv : byclass) dense_bytes += kv.second.size() * kv.second.size() * 12;
   if (dense_bytes > ((size_t)2 << 30))
     return fail(COSMO_EUNSUP, "factorised operator: single-species blocks too large for dense products");
-  std::vector<int64_t> poff(nblk, 0);
+  std::vector<int64_t> poff(nblk, 0), qoff(nblk, 0);
+  std::vector<int32_t> qnb(nblk, 0);
   int64_t prow = 0;
   for (auto &kv : byclass) {
     System sub = s;
@@ -600,10 +616,13 @@ int build_factored(Op &op, const System &s) {
     c.nb = N;
     c.S = (int)(s.noff[kv.second[0] + 1] - s.noff[kv.second[0]]);
     c.prow = prow;
-    for (int b : kv.second) {
-      poff[b] = prow;
-      prow += c.S;
+    for (int i = 0; i < N; ++i) {
+      const int b = kv.second[i];
+      poff[b] = prow + (int64_t)i * c.S;
+      qoff[b] = prow + i;
+      qnb[b] = N;
     }
+    prow += (int64_t)N * c.S;
     const int sec = s.nsector[kv.second[0]];
     std::vector<float> Pf(hp_dense[sec].begin(), hp_dense[sec].end());
     if ((rc = f.put(Hd, &c.H64)) || (rc = f.put(Hf, &c.H32)) || (rc = f.put(hp_dense[sec], &c.P64)) ||
@@ -622,7 +641,7 @@ int build_factored(Op &op, const System &s) {
   A.wneg = wneg;
   int rc;
   if ((rc = f.put(s.noff, &A.noff)) || (rc = f.put(s.nsector, &A.nsector)) || (rc = f.put(s.nmask, &A.nmask)) ||
-      (rc = f.put(poff, &A.poff)) || (rc = f.put(step_start, &A.step_start)) || (rc = f.put(step_tb, &A.step_tb)) ||
+      (rc = f.put(poff, &A.poff)) || (rc = f.put(qoff, &A.qoff)) || (rc = f.put(qnb, &A.qnb)) || (rc = f.put(step_start, &A.step_start)) || (rc = f.put(step_tb, &A.step_tb)) ||
       (rc = f.put(step_code, &A.step_code)) || (rc = f.put(hk_pbeg, &A.hk_pbeg)) || (rc = f.put(hk_ptr, &A.hk_ptr)) ||
       (rc = f.put(hk_words, &A.hk_words)) || (rc = f.put(W, &f.W64)) || (rc = f.put(W32, &f.W32)))
     return rc;
@@ -639,18 +658,18 @@ void free_factored(Op &op) {
 
 size_t factored_bytes(const Op &op) {
   const FactOp *f = static_cast<const FactOp *>(op.fact);
-  return f ? f->bytes + 2 * f->p_bytes : 0;
+  return f ? f->bytes + 4 * f->p_bytes : 0;
 }
 
 template <typename T, int LPE>
 static bool launch_fact_r(int rmax, dim3 g, size_t lds, hipStream_t st, const FactArgs &A, const T *W, const T *X,
-                          const T *Yp, T *Y, int ld, int m) {
+                          const T *Yp, const T *Yq, T *Y, int ld, int m) {
 #define FACT_R(R)                                                                                    \
   if (rmax <= R) {                                                                                   \
     auto kern = k_fact<T, LPE, R>;                                                                   \
     if (lds > 65536)                                                                                 \
       hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-    hipLaunchKernelGGL(kern, g, dim3(FACT_THREADS), lds, st, A, W, X, Yp, Y, ld, m);                \
+    hipLaunchKernelGGL(kern, g, dim3(FACT_THREADS), lds, st, A, W, X, Yp, Yq, Y, ld, m);            \
     return true;                                                                                     \
   }
   FACT_R(1)
@@ -691,34 +710,40 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
   if (lds > 160 * 1024 || f.np2 > 2 * FACT_THREADS)
     return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
   // single-species parts on the class-contiguous copy of X:
-  //   Yp_class = Xp_class Hn_class            (one GEMM, columns = neutron blocks)
-  //   Yp_blk  += Xp_blk Hp_sector per block   (strided batched GEMM, blocks of S rows)
+  //   Yp_class = Xp_class Hn_class   (neutron-major: S*ld x nb, one column per block)
+  //   Yq_class = Xq_class Hp_sector  (proton-major: nb*ld x S, one column per proton row)
   const size_t need = (size_t)f.n * ld * sizeof(T);
   if (f.p_bytes < need) {
-    hipFree(f.Xp);
-    hipFree(f.Yp);
-    f.Xp = f.Yp = nullptr;
+    dfree(f.Xp);
+    dfree(f.Yp);
+    dfree(f.Xq);
+    dfree(f.Yq);
+    f.Xp = f.Yp = f.Xq = f.Yq = nullptr;
     f.p_bytes = 0;
-    HIPCHK(hipMalloc(&f.Xp, need));
-    HIPCHK(hipMalloc(&f.Yp, need));
+    HIPCHK(dmalloc(&f.Xp, need, st));
+    HIPCHK(dmalloc(&f.Yp, need, st));
+    HIPCHK(dmalloc(&f.Xq, need, st));
+    HIPCHK(dmalloc(&f.Yq, need, st));
     f.p_bytes = need;
   }
-  T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp;
-  hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)A.nblk), dim3(256), 0, st, A.nblk, A.noff, A.poff, X, Xp, ld);
+  T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp, *Xq = (T *)f.Xq, *Yq = (T *)f.Yq;
+  hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)A.nblk), dim3(256), 0, st, A.nblk, A.noff, A.poff, A.qoff,
+                     A.qnb, X, Xp, Xq, ld);
   HIPCHK(hipGetLastError());
   if (rocblas_set_stream(f.blas, st) != rocblas_status_success) return fail(COSMO_EHIP, "rocblas_set_stream failed");
   const T one = 1, zero = 0;
   const auto N_ = rocblas_operation_none;
   for (auto &c : f.cls) {
     const rocblas_int L = (rocblas_int)((int64_t)c.S * ld), N = c.nb, S = c.S;
-    T *xc = Xp + c.prow * ld, *yc = Yp + c.prow * ld;
+    const rocblas_int M = (rocblas_int)((int64_t)c.nb * ld);
+    T *xc = Xp + c.prow * ld, *yc = Yp + c.prow * ld, *xq = Xq + c.prow * ld, *yq = Yq + c.prow * ld;
     rocblas_status b1, b2;
     if constexpr (sizeof(T) == 8) {
       b1 = rocblas_dgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H64, N, &zero, yc, L);
-      b2 = rocblas_dgemm_strided_batched(f.blas, N_, N_, ld, S, S, &one, xc, ld, L, c.P64, S, 0, &one, yc, ld, L, N);
+      b2 = rocblas_dgemm(f.blas, N_, N_, M, S, S, &one, xq, M, c.P64, S, &zero, yq, M);
     } else {
       b1 = rocblas_sgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H32, N, &zero, yc, L);
-      b2 = rocblas_sgemm_strided_batched(f.blas, N_, N_, ld, S, S, &one, xc, ld, L, c.P32, S, 0, &one, yc, ld, L, N);
+      b2 = rocblas_sgemm(f.blas, N_, N_, M, S, S, &one, xq, M, c.P32, S, &zero, yq, M);
     }
     if (b1 != rocblas_status_success || b2 != rocblas_status_success)
       return fail(COSMO_EHIP, "factorised apply: rocblas gemm failed");
@@ -726,9 +751,9 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
   const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
   dim3 g((unsigned)A.nblk, (unsigned)slices);
   bool ok;
-  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
-  else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
-  else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Y, ld, m);
+  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
   if (!ok) return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
   HIPCHK(hipGetLastError());
   return COSMO_OK;

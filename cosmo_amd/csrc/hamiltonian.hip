@@ -50,6 +50,28 @@ hipStream_t lib_stream() {
   return t_stream[d];
 }
 
+static bool g_pool_done[64] = {};
+
+hipError_t dmalloc_void(void **p, size_t bytes, hipStream_t st) {
+  const int d = t_device;
+  if (!g_pool_done[d]) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_pool_done[d]) {
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;  // freed blocks stay in the pool for the next solve
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      }
+      g_pool_done[d] = true;
+    }
+  }
+  return hipMallocAsync(p, bytes ? bytes : 1, st ? st : lib_stream());
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFreeAsync(p, lib_stream());
+}
+
 void release_thread_streams() {
   for (int d = 0; d < 64; ++d)
     if (t_stream[d]) {
@@ -687,7 +709,7 @@ int upload_tables(const System &s, DevTablesOwner &o, hipStream_t st) {
   for (auto &p : pcs)
     if (p.bytes) memcpy(host.data() + p.off, p.src, p.bytes);
   void *blob = nullptr;
-  HIPCHK(hipMalloc(&blob, total));
+  HIPCHK(dmalloc(&blob, total));
   HIPCHK(hipMemcpyAsync(blob, host.data(), total, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   auto at = [&](size_t off) { return (const void *)((const char *)blob + off); };
@@ -727,7 +749,7 @@ int upload_tables(const System &s, DevTablesOwner &o, hipStream_t st) {
 }
 
 void free_tables(DevTablesOwner &o) {
-  if (o.blob) hipFree(o.blob);
+  if (o.blob) dfree(o.blob);
   o.blob = nullptr;
 }
 
@@ -749,8 +771,8 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
   const int keep = (flags & COSMO_OP_KEEP_SMALL) ? 1 : 0;
   int32_t *counts = nullptr;
   int *err = nullptr;
-  HIPCHK(hipMalloc(&counts, sizeof(int32_t) * (n + 1)));
-  HIPCHK(hipMalloc(&err, sizeof(int)));
+  HIPCHK(dmalloc(&counts, sizeof(int32_t) * (n + 1)));
+  HIPCHK(dmalloc(&err, sizeof(int)));
   HIPCHK(hipMemsetAsync(err, 0, sizeof(int), st));
   CsrDev &M = op.csr;
   int cap = 2048;
@@ -760,8 +782,8 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
   HIPCHK(hipStreamSynchronize(st));
   if (herr > 0) {
     if (herr > 8192) {
-      hipFree(counts);
-      hipFree(err);
+      dfree(counts);
+      dfree(err);
       return fail(COSMO_EUNSUP, "Hamiltonian row longer than 8192 entries");
     }
     cap = 8192;
@@ -776,19 +798,19 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
   const int64_t nnz = rp[n];
   M.n = n;
   M.nnz = nnz;
-  HIPCHK(hipMalloc(&M.rowptr, sizeof(int64_t) * (n + 1)));
-  HIPCHK(hipMalloc(&M.cols, sizeof(uint32_t) * std::max<int64_t>(nnz, 1)));
-  HIPCHK(hipMalloc(&M.v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
-  if (!(flags & COSMO_OP_NO_FP32)) HIPCHK(hipMalloc(&M.v32, sizeof(float) * std::max<int64_t>(nnz, 1)));
-  HIPCHK(hipMalloc(&M.diag, sizeof(double) * std::max<int64_t>(n, 1)));
+  HIPCHK(dmalloc(&M.rowptr, sizeof(int64_t) * (n + 1)));
+  HIPCHK(dmalloc(&M.cols, sizeof(uint32_t) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(dmalloc(&M.v64, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  if (!(flags & COSMO_OP_NO_FP32)) HIPCHK(dmalloc(&M.v32, sizeof(float) * std::max<int64_t>(nnz, 1)));
+  HIPCHK(dmalloc(&M.diag, sizeof(double) * std::max<int64_t>(n, 1)));
   HIPCHK(hipMemsetAsync(M.diag, 0, sizeof(double) * n, st));
   HIPCHK(hipMemcpyAsync(M.rowptr, rp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
   if (cap == 2048) launch_build<2048>(op.tab.t, n, 1, keep, M, counts, err, st);
   else launch_build<8192>(op.tab.t, n, 1, keep, M, counts, err, st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
-  hipFree(counts);
-  hipFree(err);
+  dfree(counts);
+  dfree(err);
   // upper-triangle count: every row holds its diagonal, off-diagonals come in pairs
   M.nnz_upper = (nnz + n) / 2;
   return COSMO_OK;
@@ -968,8 +990,8 @@ int stored_footprint(const System &s, int flags, int64_t *bytes) {
   int32_t *counts = nullptr;
   int *err = nullptr;
   CsrDev M;
-  if (hipMalloc(&counts, sizeof(int32_t) * (n + 1)) != hipSuccess || hipMalloc(&err, sizeof(int)) != hipSuccess) {
-    hipFree(counts);
+  if (dmalloc(&counts, sizeof(int32_t) * (n + 1)) != hipSuccess || dmalloc(&err, sizeof(int)) != hipSuccess) {
+    dfree(counts);
     free_tables(tab);
     return fail(COSMO_ENOMEM, "footprint count buffers");
   }
@@ -986,8 +1008,8 @@ int stored_footprint(const System &s, int flags, int64_t *bytes) {
   std::vector<int32_t> hc(n);
   hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
   hipError_t e = hipStreamSynchronize(st);
-  hipFree(counts);
-  hipFree(err);
+  dfree(counts);
+  dfree(err);
   int64_t nnz = 0;
   for (int64_t r = 0; r < n; ++r) nnz += hc[r];
   *bytes = (int64_t)tab.bytes + 8 * (n + 1) + 8 * n + nnz * (4 + 8 + ((flags & COSMO_OP_NO_FP32) ? 0 : 4));

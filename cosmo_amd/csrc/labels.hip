@@ -73,7 +73,7 @@ template <typename T>
 static int up(std::vector<void *> &owned, const std::vector<T> &h, const T **out) {
   void *d = nullptr;
   size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
-  HIPCHK(hipMalloc(&d, b));
+  HIPCHK(dmalloc(&d, b));
   owned.push_back(d);
   if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   *out = (const T *)d;
@@ -84,7 +84,7 @@ static int raise_norms(const System &s, const System &t, const std::vector<int16
                        const std::vector<double> &coef, const double *Z, int count, double *out) {
   std::vector<void *> owned;
   auto cleanup = [&]() {
-    for (void *p : owned) hipFree(p);
+    for (void *p : owned) dfree(p);
   };
   if (t.dim == 0) {
     for (int v = 0; v < count; ++v) out[v] = 0.0;
@@ -122,8 +122,8 @@ static int raise_norms(const System &s, const System &t, const std::vector<int16
   rc |= up(owned, t.noff, &T.noff);
   double *dZ = nullptr, *dY = nullptr;
   if (!rc) {
-    if (hipMalloc(&dZ, sizeof(double) * s.dim * count) != hipSuccess ||
-        hipMalloc(&dY, sizeof(double) * t.dim * count) != hipSuccess)
+    if (dmalloc(&dZ, sizeof(double) * s.dim * count) != hipSuccess ||
+        dmalloc(&dY, sizeof(double) * t.dim * count) != hipSuccess)
       rc = fail(COSMO_ENOMEM, "label buffers");
   }
   if (!rc) {
@@ -140,8 +140,8 @@ static int raise_norms(const System &s, const System &t, const std::vector<int16
       out[v] = acc;
     }
   }
-  hipFree(dZ);
-  hipFree(dY);
+  dfree(dZ);
+  dfree(dY);
   cleanup();
   return rc;
 }

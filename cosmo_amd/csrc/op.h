@@ -26,6 +26,8 @@ struct Op {
   void *sx = nullptr, *sy = nullptr, *sx32 = nullptr, *sy32 = nullptr;
   size_t s_elems = 0;
   void *fact = nullptr;        // product-form tables (COSMO_OP_FACTORED)
+  hipStream_t home = nullptr;  // library stream of the creating thread
+  bool foreign = false;        // used on another stream: destroy synchronises the device
   double build_ms = 0, last_apply_ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -40,6 +42,10 @@ int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
 int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);
 int set_device_impl(int d);
+// record that op is about to be used on stream st
+inline void note_stream(Op &op, hipStream_t st) {
+  if (st != op.home) op.foreign = true;
+}
 int current_device();
 
 }  // namespace cosmo

@@ -17,17 +17,17 @@ static void free_op(Op *op) {
   if (!op) return;
   hipSetDevice(op->device);
   CsrDev &M = op->csr;
-  hipFree(M.rowptr);
-  hipFree(M.cols);
-  hipFree(M.v64);
-  hipFree(M.v32);
-  hipFree(M.diag);
+  dfree(M.rowptr);
+  dfree(M.cols);
+  dfree(M.v64);
+  dfree(M.v32);
+  dfree(M.diag);
   free_tables(op->tab);
   free_factored(*op);
-  hipFree(op->sx);
-  hipFree(op->sy);
-  hipFree(op->sx32);
-  hipFree(op->sy32);
+  dfree(op->sx);
+  dfree(op->sy);
+  dfree(op->sx32);
+  dfree(op->sy32);
   if (op->e0) hipEventDestroy(op->e0);
   if (op->e1) hipEventDestroy(op->e1);
 }
@@ -42,15 +42,15 @@ static int fetch_diag(Op &op, hipStream_t st) {
 
 static int ensure_staging(Op &op, size_t elems) {
   if (elems <= op.s_elems) return COSMO_OK;
-  hipFree(op.sx);
-  hipFree(op.sy);
-  hipFree(op.sx32);
-  hipFree(op.sy32);
+  dfree(op.sx);
+  dfree(op.sy);
+  dfree(op.sx32);
+  dfree(op.sy32);
   op.sx = op.sy = op.sx32 = op.sy32 = nullptr;
-  HIPCHK(hipMalloc(&op.sx, elems * sizeof(double)));
-  HIPCHK(hipMalloc(&op.sy, elems * sizeof(double)));
-  HIPCHK(hipMalloc(&op.sx32, elems * sizeof(float)));
-  HIPCHK(hipMalloc(&op.sy32, elems * sizeof(float)));
+  HIPCHK(dmalloc(&op.sx, elems * sizeof(double)));
+  HIPCHK(dmalloc(&op.sy, elems * sizeof(double)));
+  HIPCHK(dmalloc(&op.sx32, elems * sizeof(float)));
+  HIPCHK(dmalloc(&op.sy32, elems * sizeof(float)));
   op.s_elems = elems;
   return COSMO_OK;
 }
@@ -102,10 +102,10 @@ static int upload_upper(Op &op, int64_t n, const int64_t *rowptr, const int64_t 
   int64_t nup = 0;
   for (int64_t r = 0; r < n; ++r) nup += up[r] - fill[r];
   M.nnz_upper = nup;
-  HIPCHK(hipMalloc(&M.rowptr, sizeof(int64_t) * (n + 1)));
-  HIPCHK(hipMalloc(&M.cols, sizeof(uint32_t) * hc.size()));
-  HIPCHK(hipMalloc(&M.v64, sizeof(double) * hv.size()));
-  HIPCHK(hipMalloc(&M.diag, sizeof(double) * std::max<int64_t>(n, 1)));
+  HIPCHK(dmalloc(&M.rowptr, sizeof(int64_t) * (n + 1)));
+  HIPCHK(dmalloc(&M.cols, sizeof(uint32_t) * hc.size()));
+  HIPCHK(dmalloc(&M.v64, sizeof(double) * hv.size()));
+  HIPCHK(dmalloc(&M.diag, sizeof(double) * std::max<int64_t>(n, 1)));
   HIPCHK(hipMemcpyAsync(M.rowptr, cnt.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(M.cols, hc.data(), sizeof(uint32_t) * hc.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(M.v64, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, st));
@@ -113,7 +113,7 @@ static int upload_upper(Op &op, int64_t n, const int64_t *rowptr, const int64_t 
   if (!(flags & COSMO_OP_NO_FP32)) {
     std::vector<float> hf(hv.size());
     for (size_t i = 0; i < hv.size(); ++i) hf[i] = (float)hv[i];
-    HIPCHK(hipMalloc(&M.v32, sizeof(float) * hf.size()));
+    HIPCHK(dmalloc(&M.v32, sizeof(float) * hf.size()));
     HIPCHK(hipMemcpyAsync(M.v32, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -140,13 +140,14 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
   op->flags = flags;
   op->is_virtual = (flags & (COSMO_OP_VIRTUAL | COSMO_OP_FACTORED)) != 0;
   hipStream_t st = lib_stream();
+  op->home = st;
   auto t0 = std::chrono::steady_clock::now();
   int rc = upload_tables(*s, op->tab, st);
   if (!rc && (flags & COSMO_OP_FACTORED)) rc = build_factored(*op, *s);
   if (!rc) {
     if (op->is_virtual) {
       op->csr.n = op->n;
-      if (hipMalloc(&op->csr.diag, sizeof(double) * std::max<int64_t>(op->n, 1)) != hipSuccess)
+      if (dmalloc(&op->csr.diag, sizeof(double) * std::max<int64_t>(op->n, 1)) != hipSuccess)
         rc = fail(COSMO_ENOMEM, "diagonal allocation failed");
       else
         rc = virtual_diag(op->tab.t, op->n, op->csr.diag, st);
@@ -176,6 +177,7 @@ int cosmo_op_create_upper(int64_t n, const int64_t *rowptr, const int64_t *cols,
   op->n = n;
   op->flags = flags;
   hipStream_t st = lib_stream();
+  op->home = st;
   auto t0 = std::chrono::steady_clock::now();
   int rc = upload_upper(*op, n, rowptr, cols, vals, flags, st);
   if (rc) {
@@ -222,6 +224,11 @@ int cosmo_op_read_hh(const char *path, int flags, cosmo_op *out) {
 }
 
 int cosmo_op_destroy(cosmo_op op) {
+  if (!op) return COSMO_OK;
+  hipSetDevice(op->device);
+  // stream-ordered frees follow the work of this thread's stream; anything the operator
+  // did on other streams must be finished first
+  if (op->foreign || op->home != lib_stream()) hipDeviceSynchronize();
   free_op(op);
   delete op;
   return COSMO_OK;
@@ -254,6 +261,7 @@ int cosmo_op_apply_device(cosmo_op op, const void *X, void *Y, int m, int ld, in
   if (!op || (!X && m) || (!Y && m) || m < 0 || ld < m) return fail(COSMO_EINVAL, "cosmo_op_apply_device: bad args");
   hipSetDevice(op->device);
   hipStream_t st = stream ? (hipStream_t)stream : lib_stream();
+  note_stream(*op, st);
   hipEventRecord(op->e0, st);
   int rc = op_apply_dev(*op, X, Y, m, ld, ld, precision, st);
   hipEventRecord(op->e1, st);
@@ -265,6 +273,7 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
   if (m == 0) return COSMO_OK;
   hipSetDevice(op->device);
   hipStream_t st = lib_stream();
+  note_stream(*op, st);
   const int64_t n = op->n;
   // the product-form apply wants a power-of-two row pitch of at least 16 bytes
   int ld = m;
