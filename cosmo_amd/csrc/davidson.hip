@@ -652,8 +652,15 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   int nloops = 0, ierr = 0, restarts = 0;
   // mixed precision: collapse to the Ritz vectors, recompute H.B in FP64, restart
   auto switch_fp64 = [&]() -> int {
-    // keep the whole FP32-grown basis; only H.B and S are recomputed in FP64
-    int r = apply_block(D, 0, kpass, COSMO_PREC_FP64);
+    // keep the whole FP32-grown basis; it is re-orthonormalised in FP64 (vectors added
+    // with a single round may have drifted, which matters once the basis nears the whole
+    // space), then H.B and S are recomputed in FP64
+    bool rs0;
+    int kept0 = kpass;
+    int r = orthonormalise(D, 0, D.B, kpass, p.ortho, rs0, kept0, 2);
+    if (r) return r;
+    kpass = kept0;
+    r = apply_block(D, 0, kpass, COSMO_PREC_FP64);
     if (r) return r;
     std::fill(s.begin(), s.end(), 0.0);
     r = update_S(D, s, 0, kpass);

@@ -86,3 +86,20 @@ def test_ee_roundtrip(gpu, golden, tmp_path):
     # warm start from the written vectors converges at once to the same values
     ev2 = gpu.davidson(op, 4, init=vec, want_vectors=False)[0]
     assert np.max(np.abs(ev2 - ev)) < 1e-9
+
+
+@pytest.mark.parametrize("seed", [3, 17, 41])
+def test_mixed_precision_full_space_basis(gpu, tmp_path, seed):
+    """A basis that grows to the whole space (n = 25, 5 pairs, lim = n): the FP32 phase's
+    vectors must be re-orthonormalised in FP64 before the refinement, or the last pairs
+    stall above the residual target."""
+    from cosmo_amd import synth
+    xml = synth.write_system(str(tmp_path), "sd", "t", 1, 1, -2, "+", seed)
+    op = gpu.Operator.build(gpu.System.from_xml(xml))
+    assert op.n == 25
+    kw = dict(want_vectors=False, critr=1e-6, lim=25, maxiter=4000)
+    e64, _, r64, _, _ = gpu.davidson(op, 5, **kw)
+    e32, _, r32, _, st = gpu.davidson(op, 5, precision=gpu.PREC_FP32, **kw)
+    assert st["converged"]
+    assert np.max(r32) <= 1e-6
+    assert np.max(np.abs(e32 - e64)) < 1e-8
