@@ -7,16 +7,19 @@
 // work (H.X, Gram blocks, basis rotations, residuals, preconditioning) stays on the GPU;
 // the (lim x lim) Rayleigh-Ritz problem is solved in FP64 on the host.
 //
-// Orthogonalisation is blocked: the new vectors are projected against the existing
-// basis with two classical Gram-Schmidt sweeps (B^T V and V - B W as two tall-skinny
-// passes) and then orthonormalised among themselves by a pivot-dropping Cholesky QR,
-// repeated once in the FP64 phase (not while H.X runs in FP32).  This spans the same space as the reference's per-vector repeated CGS
-// (Orthogonalize_Basis :313-390) in a few passes over B instead of one per new vector.
+// Orthogonalisation is blocked: the new vectors V sit right after the basis B in storage,
+// so one Gram pass gives both B^T V and V^T V; classical Gram-Schmidt sweeps (V - B W)
+// repeat until the overlap is below the threshold, and a pivot-dropping Cholesky QR of
+// V^T V - W^T W orthonormalises V among themselves; the whole is repeated once in the
+// FP64 phase (not while H.X runs in FP32).  This spans the same space as the
+// reference's per-vector repeated CGS (Orthogonalize_Basis :313-390) in a few passes
+// over B instead of one per new vector.  Gram blocks run on the FP64 matrix cores.
 //
-// Mixed precision (COSMO_PREC_FP32): H.X runs on the FP32 values with FP32 vectors until
-// every wanted residual drops under mixed_switch * max(1, |lambda|); the basis is then
-// collapsed to the Ritz vectors, H.X is recomputed in FP64 and the iteration continues
-// in FP64 to the reference thresholds (the FP64 refinement).
+// Mixed precision (COSMO_PREC_FP32): H.X runs on the FP32 values with FP32 vectors, and
+// the basis, its image and the work blocks are stored in FP32 (all sums in FP64), until
+// every wanted residual drops under mixed_switch * max(1, |lambda|); the whole basis is
+// then widened to FP64 and re-orthonormalised, H.X is recomputed in FP64 and the
+// iteration continues in FP64 to the reference thresholds (the FP64 refinement).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,44 +35,123 @@
 namespace cosmo {
 bool dense_symmetric_eigen(int n, const std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
 
-#define GRAM_ROWS 32
-#define GRAM_MAXW 160  // p + q limit of one Gram launch
 
-// partial[g][a*q+b] = sum over the rows of block g of A[:,a] * Bm[:,b]
-__global__ __launch_bounds__(256) void k_gram(int64_t n, const double *__restrict__ A, int64_t lda, int p,
-                                              const double *__restrict__ Bm, int64_t ldb, int q,
-                                              double *__restrict__ partial) {
-  __shared__ double sa[GRAM_ROWS * GRAM_MAXW];
-  double acc[4] = {0, 0, 0, 0};
-  const int pq = p * q;
-  const int w = p + q;
-  for (int64_t base = (int64_t)blockIdx.x * GRAM_ROWS; base < n; base += (int64_t)gridDim.x * GRAM_ROWS) {
-    const int rows = (int)min<int64_t>(GRAM_ROWS, n - base);
-    __syncthreads();
-    for (int t = threadIdx.x; t < GRAM_ROWS * w; t += blockDim.x) {
-      int col = t / GRAM_ROWS, r = t % GRAM_ROWS;
-      double v = 0.0;
-      if (r < rows) v = (col < p) ? A[(int64_t)col * lda + base + r] : Bm[(int64_t)(col - p) * ldb + base + r];
-      sa[col * GRAM_ROWS + r] = v;
-    }
-    __syncthreads();
+#define GRAM_MAXP 128   // A columns per launch (8 output tiles of 16)
+#define GRAM_MAXQ 16    // Bm columns per launch (one output tile)
+#define GRAM_SLAB 32    // rows per wave per step: 4 lane groups x 8 rows
+#define GRAM_LDS 33     // LDS column stride of a staged 32-row tile (doubles)
+#define GRAM_BLOCKS 1024
+#define GRAM_SMALL 8192  // doubles in the small device buffer (Gram results, coefficients)
+
+// partial[g][a*q+b] = sum over the rows that block g visits of A[:,a] * Bm[:,b], on the
+// FP64 matrix cores (v_mfma_f64_16x16x4f64: 16 x 16 output tile, K = 4 rows per
+// instruction).  Each wave takes 32-row slabs round-robin over the grid.  A slab of 16
+// columns is read coalesced (32 consecutive rows of a column per half-wave), widened
+// exactly to FP64 and staged in the wave's own LDS tile; lane (i, kg) = (lane % 16,
+// lane / 16) then feeds column i at rows kg*8 + j, j = 0..7, as the 8 K steps.  The Bm
+// tile is staged once per slab, the A tiles one after another.  The four waves of a block
+// are summed in a fixed order through LDS, so the result does not depend on timing.
+// Columns past p (q) are read clamped and their outputs dropped; rows past n read as zero.
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+
+// orders this wave's LDS accesses: every earlier LDS access has completed, and the
+// compiler keeps memory accesses on their side
+__device__ __forceinline__ void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// 16 columns x 32 rows of a slab: lane (row, half) = (lane % 32, lane / 32) reads its row
+// of columns 2s + half, s = 0..7 (two whole column segments per instruction).  p0 points
+// at the lane's row in column 0 of the tile, p1 at the same row in column `half`; the
+// column offsets are uniform, and columns past ncol read column ncol - 1 (or 0), unused.
+template <typename V>
+__device__ __forceinline__ void gram_load(V (&v)[8], const V *p0, const V *p1, int64_t ld, int ncol) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      int o = threadIdx.x + k * 256;
-      if (o < pq) {
-        int a = o / q, b = o % q;
-        const double *x = sa + a * GRAM_ROWS, *y = sa + (p + b) * GRAM_ROWS;
-        double s = 0.0;
-#pragma unroll 8
-        for (int r = 0; r < GRAM_ROWS; ++r) s += x[r] * y[r];
-        acc[k] += s;
+  for (int s = 0; s < 8; ++s) {
+    const int c = 2 * s;
+    const V *src = (c + 1 < ncol) ? p1 : p0;
+    v[s] = src[(c < ncol) ? (int64_t)c * ld : 0];
+  }
+}
+template <typename V>
+__device__ __forceinline__ void gram_store(double *__restrict__ tile, const V (&v)[8], int64_t r0, int64_t n,
+                                           int lane) {
+  const int row = lane & 31, half = lane >> 5;
+  const bool ok = r0 + row < n;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) tile[(2 * s + half) * GRAM_LDS + row] = ok ? (double)v[s] : 0.0;
+}
+
+template <typename V, int PT>
+__global__ __launch_bounds__(256) void k_gram(int64_t n, const V *__restrict__ A, int64_t lda, int p,
+                                              const V *__restrict__ Bm, int64_t ldb, int q,
+                                              double *__restrict__ partial) {
+  __shared__ double red[PT * 256];
+  __shared__ double stage[4][2][16 * GRAM_LDS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, kg = lane >> 4;
+  double *tb = stage[wave][0], *ta = stage[wave][1];
+  gram_d4 acc[PT];
+#pragma unroll
+  for (int t = 0; t < PT; ++t) acc[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  // software pipeline: the next A tile (and, during the last tile, the next slab's Bm and
+  // first A tile) is in flight while the current one is multiplied; rows are clamped, so
+  // a read past the last slab stays inside the arrays and is never used
+  const int64_t nslab = (n + GRAM_SLAB - 1) / GRAM_SLAB, stride = (int64_t)gridDim.x * 4;
+  int64_t s = (int64_t)blockIdx.x * 4 + wave;
+  const int half = lane >> 5;
+  auto row_of = [&](int64_t r0) { return min(r0 + (lane & 31), n - 1); };
+  V vb[8], vc[8];
+  {
+    const int64_t r = row_of(s * GRAM_SLAB);
+    gram_load<V>(vb, Bm + r, Bm + r + half * ldb, ldb, q);
+    gram_load<V>(vc, A + r, A + r + half * lda, lda, p);
+  }
+  for (; s < nslab; s += stride) {
+    const int64_t r0 = s * GRAM_SLAB, r = row_of(r0), rn = row_of(r0 + stride * GRAM_SLAB);
+    wave_lds_fence();
+    gram_store<V>(tb, vb, r0, n, lane);
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+      gram_store<V>(ta, vc, r0, n, lane);
+      if (t + 1 < PT) {
+        const V *p0 = A + (int64_t)(t + 1) * 16 * lda + r;
+        gram_load<V>(vc, p0, p0 + half * lda, lda, p - (t + 1) * 16);
+      } else {
+        gram_load<V>(vb, Bm + rn, Bm + rn + half * ldb, ldb, q);
+        gram_load<V>(vc, A + rn, A + rn + half * lda, lda, p);
       }
+      wave_lds_fence();
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[i * GRAM_LDS + kg * 8 + j], tb[i * GRAM_LDS + kg * 8 + j],
+                                                      acc[t], 0, 0, 0);
+      wave_lds_fence();
     }
   }
+  // fixed-order block sum: ((w3 + w2) + w1) + w0
+  for (int w = 3; w >= 1; --w) {
+    if (wave == w) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int o = threadIdx.x + k * 256;
-    if (o < pq) partial[(int64_t)blockIdx.x * pq + o] = acc[k];
+      for (int t = 0; t < PT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = acc[t][v];
+          if (w < 3) x += red[t * 256 + v * 64 + lane];
+          red[t * 256 + v * 64 + lane] = x;
+        }
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+    const int pq = p * q;
+    // the FP64 16x16 result puts tile row 4*v + kg, column i in register v of lane
+    // (i, kg): element v of acc[t] is G[t*16 + 4*v + kg][i]
+#pragma unroll
+    for (int t = 0; t < PT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int a = t * 16 + 4 * v + kg;
+        if (a < p && i < q) partial[(int64_t)blockIdx.x * pq + a * q + i] = acc[t][v] + red[t * 256 + v * 64 + lane];
+      }
   }
 }
 
@@ -94,10 +176,10 @@ __global__ void k_reduce_segments(int pq, const double *__restrict__ seg, double
 }
 
 // Y[:, j] = sum_k A1[:,k] C1[k*q+j] (+ A2[:,k] C2[k*q+j]), j < q <= 16, per row
-template <int Q>
-__global__ __launch_bounds__(256) void k_gemm(int64_t n, const double *__restrict__ A1, const double *__restrict__ A2,
+template <int Q, typename V>
+__global__ __launch_bounds__(256) void k_gemm(int64_t n, const V *__restrict__ A1, const V *__restrict__ A2,
                                               int64_t lda, int p, const double *__restrict__ C1,
-                                              const double *__restrict__ C2, int q, double *__restrict__ Y,
+                                              const double *__restrict__ C2, int q, V *__restrict__ Y,
                                               int64_t ldy) {
   __shared__ double c1[128 * Q], c2[128 * Q];
   for (int t = threadIdx.x; t < p * q; t += blockDim.x) {
@@ -112,24 +194,24 @@ __global__ __launch_bounds__(256) void k_gemm(int64_t n, const double *__restric
 #pragma unroll
   for (int j = 0; j < Q; ++j) acc[j] = 0.0;
   for (int k = 0; k < p; ++k) {
-    double a = A1[(int64_t)k * lda + i];
+    double a = (double)A1[(int64_t)k * lda + i];
 #pragma unroll
     for (int j = 0; j < Q; ++j) acc[j] += a * c1[k * Q + j];
     if (A2) {
-      double b = A2[(int64_t)k * lda + i];
+      double b = (double)A2[(int64_t)k * lda + i];
 #pragma unroll
       for (int j = 0; j < Q; ++j) acc[j] += b * c2[k * Q + j];
     }
   }
 #pragma unroll
   for (int j = 0; j < Q; ++j)
-    if (j < q) Y[(int64_t)j * ldy + i] = acc[j];
+    if (j < q) Y[(int64_t)j * ldy + i] = (V)acc[j];
 }
 
-// V[:, j] -= sum_k B[:,k] W[k*q+j]
-template <int Q>
-__global__ __launch_bounds__(256) void k_project_out(int64_t n, const double *__restrict__ B, int64_t ldb, int p,
-                                                     const double *__restrict__ W, int q, double *__restrict__ V,
+// X[:, j] -= sum_k B[:,k] W[k*q+j]
+template <int Q, typename V>
+__global__ __launch_bounds__(256) void k_project_out(int64_t n, const V *__restrict__ B, int64_t ldb, int p,
+                                                     const double *__restrict__ W, int q, V *__restrict__ X,
                                                      int64_t ldv) {
   __shared__ double w[128 * Q];
   for (int t = threadIdx.x; t < p * q; t += blockDim.x) w[(t / q) * Q + t % q] = W[t];
@@ -140,51 +222,68 @@ __global__ __launch_bounds__(256) void k_project_out(int64_t n, const double *__
 #pragma unroll
   for (int j = 0; j < Q; ++j) acc[j] = 0.0;
   for (int k = 0; k < p; ++k) {
-    double b = B[(int64_t)k * ldb + i];
+    double b = (double)B[(int64_t)k * ldb + i];
 #pragma unroll
     for (int j = 0; j < Q; ++j) acc[j] += b * w[k * Q + j];
   }
 #pragma unroll
   for (int j = 0; j < Q; ++j)
-    if (j < q) V[(int64_t)j * ldv + i] -= acc[j];
+    if (j < q) X[(int64_t)j * ldv + i] = (V)((double)X[(int64_t)j * ldv + i] - acc[j]);
 }
 
-// diagonal preconditioner: r /= (lambda - D_ii), |.| floored at 1e-13 (fsdavidson.cxx:498-510)
-__global__ void k_precond(int64_t n, int q, double *__restrict__ R, int64_t ldr, const double *__restrict__ lam,
-                          const double *__restrict__ D) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * q) return;
-  int64_t i = t % n;
-  int j = (int)(t / n);
-  double dg = lam[j] - D[i];
-  double *r = R + (int64_t)j * ldr + i;
-  if (fabs(dg) > 1.0e-13) *r = *r / dg;
-  else *r = *r / 1.0e-13;
+// Up to 16 column indices (and one value per column) passed by value to a launch.
+struct ColList {
+  int c[16];
+  double s[16];
+};
+
+// dst[:, a] = src[:, L.c[a]] for a = blockIdx.y < 16 (column-major, ld n)
+template <typename V>
+__global__ void k_copy_cols(int64_t n, const V *__restrict__ src, ColList L, V *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = blockIdx.y;
+  dst[(int64_t)a * n + i] = src[(int64_t)L.c[a] * n + i];
 }
 
-__global__ void k_scale_cols(int64_t n, int q, double *__restrict__ V, int64_t ldv, const double *__restrict__ sc) {
+// dst[:, a] = src[:, L.c[a]] / (L.s[a] - D_ii), |.| of the denominator floored at 1e-13
+// (diagonal preconditioner, fsdavidson.cxx:498-510): the residual of pair a moved into
+// the basis and preconditioned in one pass
+template <typename V>
+__global__ void k_precond_cols(int64_t n, const V *__restrict__ src, ColList L, V *__restrict__ dst,
+                               const double *__restrict__ D) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = blockIdx.y;
+  const double dg = L.s[a] - D[i];
+  const double r = (double)src[(int64_t)L.c[a] * n + i];
+  dst[(int64_t)a * n + i] = (V)(fabs(dg) > 1.0e-13 ? r / dg : r / 1.0e-13);
+}
+
+// element-wise copy between the FP32 and FP64 basis storage (n x m, column-major, ld n)
+template <typename D, typename S>
+__global__ void k_convert(int64_t len, const S *__restrict__ src, D *__restrict__ dst) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * q) return;
-  V[(t / n) * ldv + t % n] *= sc[t / n];
+  if (t < len) dst[t] = (D)src[t];
 }
 
 // column-major B[:, c0 .. c0+m) -> interleaved X (n x ldx), as double or float
 // (columns m .. ldx-1 of X are zero padding)
-template <typename T>
-__global__ void k_cols_to_rows(int64_t n, int m, const double *__restrict__ B, int64_t ldb, T *__restrict__ X, int ldx) {
+template <typename T, typename V>
+__global__ void k_cols_to_rows(int64_t n, int m, const V *__restrict__ B, int64_t ldb, T *__restrict__ X, int ldx) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * ldx) return;
   int64_t i = t / ldx;
   int j = (int)(t % ldx);
   X[i * ldx + j] = j < m ? (T)B[(int64_t)j * ldb + i] : (T)0;
 }
-template <typename T>
-__global__ void k_rows_to_cols(int64_t n, int m, const T *__restrict__ Y, int ldy, double *__restrict__ B, int64_t ldb) {
+template <typename T, typename V>
+__global__ void k_rows_to_cols(int64_t n, int m, const T *__restrict__ Y, int ldy, V *__restrict__ B, int64_t ldb) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * m) return;
   int j = (int)(t / n);
   int64_t i = t % n;
-  B[(int64_t)j * ldb + i] = (double)Y[i * ldy + j];
+  B[(int64_t)j * ldb + i] = (V)Y[i * ldy + j];
 }
 
 // ------------------------------------------------------------------------------------
@@ -195,118 +294,204 @@ struct Dav {
   int lim = 0, nume = 0, mblock = 0;
   double *B = nullptr, *AB = nullptr, *R = nullptr, *T = nullptr, *Xd = nullptr, *Yd = nullptr;
   float *Xf = nullptr, *Yf = nullptr;
-  double *partial = nullptr, *segs = nullptr, *small = nullptr, *small2 = nullptr, *lam = nullptr;
+  // While H.X runs in FP32 the basis, its H image and the work blocks are kept as FP32
+  // copies (half the bytes for every pass over them; products still accumulate in FP64);
+  // es is the element size of the current phase and Bv/ABv/Rv/Tv point at its storage.
+  float *B32 = nullptr, *AB32 = nullptr, *R32 = nullptr, *T32 = nullptr;
+  int es = 8;
+  void *Bv = nullptr, *ABv = nullptr, *Rv = nullptr, *Tv = nullptr;
+  double *partial = nullptr, *segs = nullptr, *small = nullptr;
+  double *hbuf = nullptr;  // pinned host staging of GRAM_SMALL doubles (per host thread, kept)
   int G = 0;
   double ms_apply = 0, ms_ortho = 0, ms_rr = 0;
   int matvecs = 0, matvecs64 = 0;
   ~Dav() {
-    dfree(B);
-    dfree(AB);
-    dfree(R);
-    dfree(T);
-    dfree(Xd);
-    dfree(Yd);
-    dfree(Xf);
-    dfree(Yf);
-    dfree(partial);
-    dfree(segs);
-    dfree(small);
-    dfree(small2);
-    dfree(lam);
+    for (void *p : {(void *)B, (void *)AB, (void *)R, (void *)T, (void *)Xd, (void *)Yd, (void *)Xf, (void *)Yf,
+                    (void *)B32, (void *)AB32, (void *)R32, (void *)T32, (void *)partial, (void *)segs,
+                    (void *)small})
+      dfree(p);
   }
   double *col(double *base, int j) { return base + (int64_t)j * n; }
+  void *cv(void *base, int j) const { return (char *)base + (int64_t)j * n * es; }
+  void use64() {
+    es = 8;
+    Bv = B;
+    ABv = AB;
+    Rv = R;
+    Tv = T;
+  }
+  void use32() {
+    es = 4;
+    Bv = B32;
+    ABv = AB32;
+    Rv = R32;
+    Tv = T32;
+  }
 };
 
-static int dav_alloc(Dav &D) {
+static int dav_alloc(Dav &D, bool mixed) {
   const int64_t n = D.n;
+  const int nr = std::max(D.nume, D.mblock), nt = std::max(D.lim, 16);
   HIPCHK(dmalloc(&D.B, sizeof(double) * n * D.lim));
   HIPCHK(dmalloc(&D.AB, sizeof(double) * n * D.lim));
-  HIPCHK(dmalloc(&D.R, sizeof(double) * n * std::max(D.nume, D.mblock)));
-  HIPCHK(dmalloc(&D.T, sizeof(double) * n * std::max(D.lim, 16)));
+  HIPCHK(dmalloc(&D.R, sizeof(double) * n * nr));
+  HIPCHK(dmalloc(&D.T, sizeof(double) * n * nt));
+  if (mixed) {
+    HIPCHK(dmalloc(&D.B32, sizeof(float) * n * D.lim));
+    HIPCHK(dmalloc(&D.AB32, sizeof(float) * n * D.lim));
+    HIPCHK(dmalloc(&D.R32, sizeof(float) * n * nr));
+    HIPCHK(dmalloc(&D.T32, sizeof(float) * n * nt));
+  }
   int ldmax = 4;
   while (ldmax < D.mblock) ldmax <<= 1;
   HIPCHK(dmalloc(&D.Xd, sizeof(double) * n * ldmax));
   HIPCHK(dmalloc(&D.Yd, sizeof(double) * n * ldmax));
   HIPCHK(dmalloc(&D.Xf, sizeof(float) * n * ldmax));
   HIPCHK(dmalloc(&D.Yf, sizeof(float) * n * ldmax));
-  D.G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + GRAM_ROWS - 1) / GRAM_ROWS));
-  HIPCHK(dmalloc(&D.partial, sizeof(double) * (size_t)D.G * 1024));
-  HIPCHK(dmalloc(&D.segs, sizeof(double) * (size_t)GRAM_SEG * 1024));
-  HIPCHK(dmalloc(&D.small, sizeof(double) * 8192));
-  HIPCHK(dmalloc(&D.small2, sizeof(double) * 8192));
-  HIPCHK(dmalloc(&D.lam, sizeof(double) * 256));
+  D.G = (int)std::min<int64_t>(GRAM_BLOCKS, std::max<int64_t>(1, (n + 4 * GRAM_SLAB - 1) / (4 * GRAM_SLAB)));
+  HIPCHK(dmalloc(&D.partial, sizeof(double) * (size_t)D.G * GRAM_MAXP * GRAM_MAXQ));
+  HIPCHK(dmalloc(&D.segs, sizeof(double) * (size_t)GRAM_SEG * GRAM_MAXP * GRAM_MAXQ));
+  HIPCHK(dmalloc(&D.small, sizeof(double) * GRAM_SMALL));
+  // the pinned staging buffer belongs to the host thread and is never freed: freeing
+  // pinned memory synchronises the device, which would stall concurrent solves
+  static thread_local double *hstage = nullptr;
+  if (!hstage) HIPCHK(hipHostMalloc((void **)&hstage, sizeof(double) * GRAM_SMALL, hipHostMallocDefault));
+  D.hbuf = hstage;
   HIPCHK(hipMemsetAsync(D.B, 0, sizeof(double) * n * D.lim, D.st));
   HIPCHK(hipMemsetAsync(D.AB, 0, sizeof(double) * n * D.lim, D.st));
+  D.use64();
   return COSMO_OK;
 }
 
-// out (p x q, row-major a*q+b) = A[:, 0..p)^T Bm[:, 0..q), host copy
-static int gram(Dav &D, const double *A, int p, const double *Bm, int q, std::vector<double> &out) {
+// copy m basis columns between the FP64 and FP32 storage (dst/src element types by es)
+static int convert_cols(Dav &D, const void *src, int src_es, void *dst, int m) {
+  const int64_t len = D.n * m;
+  if (!len) return COSMO_OK;
+  dim3 g((unsigned)((len + 255) / 256)), b(256);
+  if (src_es == 8) hipLaunchKernelGGL((k_convert<float, double>), g, b, 0, D.st, len, (const double *)src, (float *)dst);
+  else hipLaunchKernelGGL((k_convert<double, float>), g, b, 0, D.st, len, (const float *)src, (double *)dst);
+  HIPCHK(hipGetLastError());
+  return COSMO_OK;
+}
+
+template <typename V>
+static void launch_gram(Dav &D, const V *A, int pp, const V *Bm, int qq) {
+  dim3 g(D.G), b(256);
+  switch ((pp + 15) / 16) {
+    case 1: hipLaunchKernelGGL((k_gram<V, 1>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 2: hipLaunchKernelGGL((k_gram<V, 2>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 3: hipLaunchKernelGGL((k_gram<V, 3>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 4: hipLaunchKernelGGL((k_gram<V, 4>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 5: hipLaunchKernelGGL((k_gram<V, 5>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 6: hipLaunchKernelGGL((k_gram<V, 6>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    case 7: hipLaunchKernelGGL((k_gram<V, 7>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+    default: hipLaunchKernelGGL((k_gram<V, 8>), g, b, 0, D.st, D.n, A, D.n, pp, Bm, D.n, qq, D.partial); break;
+  }
+}
+
+// out (p x q, row-major a*q+b) = A[:, 0..p)^T Bm[:, 0..q), host copy; A, Bm in the
+// current phase's storage.  Launches cover at most 128 x 16 outputs; their sums are
+// gathered in the small device buffer and copied back with one synchronisation per fill.
+static int gram(Dav &D, const void *A, int p, const void *Bm, int q, std::vector<double> &out) {
   out.assign((size_t)p * q, 0.0);
   if (!p || !q) return COSMO_OK;
-  // split p so that p_chunk*q <= 1024 and p_chunk+q <= GRAM_MAXW
-  int pc = std::max(1, std::min({p, 1024 / q, GRAM_MAXW - q}));
-  std::vector<double> tmp;
-  for (int a0 = 0; a0 < p; a0 += pc) {
-    int pp = std::min(pc, p - a0);
-    hipLaunchKernelGGL(k_gram, dim3(D.G), dim3(256), 0, D.st, D.n, A + (int64_t)a0 * D.n, D.n, pp, Bm, D.n, q,
-                       D.partial);
-    hipLaunchKernelGGL(k_reduce_partial, dim3((pp * q + 255) / 256, GRAM_SEG), dim3(256), 0, D.st, D.G, pp * q,
-                       D.partial, D.segs);
-    hipLaunchKernelGGL(k_reduce_segments, dim3((pp * q + 255) / 256), dim3(256), 0, D.st, pp * q, D.segs, D.small);
-    HIPCHK(hipGetLastError());
-    tmp.resize((size_t)pp * q);
-    HIPCHK(hipMemcpyAsync(tmp.data(), D.small, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, D.st));
+  struct Piece {
+    int a0, b0, pp, qq;
+    size_t off;
+  };
+  std::vector<Piece> pend;
+  size_t used = 0;
+  auto flush = [&]() -> int {
+    if (pend.empty()) return COSMO_OK;
+    HIPCHK(hipMemcpyAsync(D.hbuf, D.small, sizeof(double) * used, hipMemcpyDeviceToHost, D.st));
     HIPCHK(hipStreamSynchronize(D.st));
-    std::copy(tmp.begin(), tmp.end(), out.begin() + (size_t)a0 * q);
+    for (const Piece &c : pend)
+      for (int a = 0; a < c.pp; ++a)
+        for (int b = 0; b < c.qq; ++b) out[(size_t)(c.a0 + a) * q + c.b0 + b] = D.hbuf[c.off + (size_t)a * c.qq + b];
+    pend.clear();
+    used = 0;
+    return COSMO_OK;
+  };
+  for (int b0 = 0; b0 < q; b0 += GRAM_MAXQ) {
+    const int qq = std::min(GRAM_MAXQ, q - b0);
+    const void *Bb = (const char *)Bm + (int64_t)b0 * D.n * D.es;
+    for (int a0 = 0; a0 < p; a0 += GRAM_MAXP) {
+      const int pp = std::min(GRAM_MAXP, p - a0), pq = pp * qq;
+      if (used + pq > GRAM_SMALL) {
+        int rc = flush();
+        if (rc) return rc;
+      }
+      const void *Aa = (const char *)A + (int64_t)a0 * D.n * D.es;
+      if (D.es == 4) launch_gram<float>(D, (const float *)Aa, pp, (const float *)Bb, qq);
+      else launch_gram<double>(D, (const double *)Aa, pp, (const double *)Bb, qq);
+      hipLaunchKernelGGL(k_reduce_partial, dim3((pq + 255) / 256, GRAM_SEG), dim3(256), 0, D.st, D.G, pq, D.partial,
+                         D.segs);
+      hipLaunchKernelGGL(k_reduce_segments, dim3((pq + 255) / 256), dim3(256), 0, D.st, pq, D.segs, D.small + used);
+      HIPCHK(hipGetLastError());
+      pend.push_back({a0, b0, pp, qq, used});
+      used += pq;
+    }
   }
-  return COSMO_OK;
+  return flush();
+}
+
+template <typename V>
+static void launch_gemm(Dav &D, int qq, const V *A1, const V *A2, int p, const double *s2, V *Yj) {
+  dim3 g((unsigned)((D.n + 255) / 256)), b(256);
+  if (qq <= 4) hipLaunchKernelGGL((k_gemm<4, V>), g, b, 0, D.st, D.n, A1, A2, D.n, p, D.small, s2, qq, Yj, D.n);
+  else if (qq <= 8) hipLaunchKernelGGL((k_gemm<8, V>), g, b, 0, D.st, D.n, A1, A2, D.n, p, D.small, s2, qq, Yj, D.n);
+  else hipLaunchKernelGGL((k_gemm<16, V>), g, b, 0, D.st, D.n, A1, A2, D.n, p, D.small, s2, qq, Yj, D.n);
 }
 
 // Y[:, 0..q) = A1 C1 (+ A2 C2); C row-major p x q on host; q <= 16 per launch
-static int gemm(Dav &D, const double *A1, const double *A2, int p, const std::vector<double> &C1,
-                const std::vector<double> *C2, int q, double *Y) {
+static int gemm(Dav &D, const void *A1, const void *A2, int p, const std::vector<double> &C1,
+                const std::vector<double> *C2, int q, void *Y) {
   if (p > 128) return fail(COSMO_EUNSUP, "basis wider than 128 vectors");
   for (int j0 = 0; j0 < q; j0 += 16) {
-    int qq = std::min(16, q - j0);
-    std::vector<double> c1((size_t)p * qq), c2;
+    // C1 then C2 chunks, packed in the pinned buffer and sent in one copy
+    const int qq = std::min(16, q - j0);
+    const size_t pc = (size_t)p * qq;
+    double *c1 = D.hbuf, *c2 = D.hbuf + pc;
     for (int k = 0; k < p; ++k)
       for (int j = 0; j < qq; ++j) c1[k * qq + j] = C1[(size_t)k * q + j0 + j];
-    if (C2) {
-      c2.resize((size_t)p * qq);
+    if (C2)
       for (int k = 0; k < p; ++k)
         for (int j = 0; j < qq; ++j) c2[k * qq + j] = (*C2)[(size_t)k * q + j0 + j];
-    }
-    HIPCHK(hipMemcpyAsync(D.small, c1.data(), sizeof(double) * c1.size(), hipMemcpyHostToDevice, D.st));
-    if (C2) HIPCHK(hipMemcpyAsync(D.small2, c2.data(), sizeof(double) * c2.size(), hipMemcpyHostToDevice, D.st));
-    dim3 g((unsigned)((D.n + 255) / 256)), b(256);
-    double *Yj = Y + (int64_t)j0 * D.n;
-    const double *s2 = C2 ? D.small2 : nullptr;
-    if (qq <= 4) hipLaunchKernelGGL(k_gemm<4>, g, b, 0, D.st, D.n, A1, C2 ? A2 : nullptr, D.n, p, D.small, s2, qq, Yj, D.n);
-    else if (qq <= 8) hipLaunchKernelGGL(k_gemm<8>, g, b, 0, D.st, D.n, A1, C2 ? A2 : nullptr, D.n, p, D.small, s2, qq, Yj, D.n);
-    else hipLaunchKernelGGL(k_gemm<16>, g, b, 0, D.st, D.n, A1, C2 ? A2 : nullptr, D.n, p, D.small, s2, qq, Yj, D.n);
+    HIPCHK(hipMemcpyAsync(D.small, D.hbuf, sizeof(double) * pc * (C2 ? 2 : 1), hipMemcpyHostToDevice, D.st));
+    const double *s2 = C2 ? D.small + pc : nullptr;
+    void *Yj = D.cv(Y, j0);
+    if (D.es == 4)
+      launch_gemm<float>(D, qq, (const float *)A1, C2 ? (const float *)A2 : nullptr, p, s2, (float *)Yj);
+    else
+      launch_gemm<double>(D, qq, (const double *)A1, C2 ? (const double *)A2 : nullptr, p, s2, (double *)Yj);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(D.st));  // small buffers are reused by the next chunk
+    HIPCHK(hipStreamSynchronize(D.st));  // the pinned buffer is refilled by the next chunk
   }
   return COSMO_OK;
 }
 
-static int project_out(Dav &D, const double *Bm, int p, const std::vector<double> &W, int q, double *V) {
+template <typename V>
+static void launch_project(Dav &D, int qq, const V *Bk, int pp, V *Vj) {
+  dim3 g((unsigned)((D.n + 255) / 256)), b(256);
+  if (qq <= 4) hipLaunchKernelGGL((k_project_out<4, V>), g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
+  else if (qq <= 8) hipLaunchKernelGGL((k_project_out<8, V>), g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
+  else hipLaunchKernelGGL((k_project_out<16, V>), g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
+}
+
+static int project_out(Dav &D, const void *Bm, int p, const std::vector<double> &W, int q, void *V) {
   if (!p || !q) return COSMO_OK;
   for (int j0 = 0; j0 < q; j0 += 16) {
     int qq = std::min(16, q - j0);
     for (int k0 = 0; k0 < p; k0 += 128) {
       int pp = std::min(128, p - k0);
-      std::vector<double> w((size_t)pp * qq);
+      double *w = D.hbuf;
       for (int k = 0; k < pp; ++k)
         for (int j = 0; j < qq; ++j) w[k * qq + j] = W[(size_t)(k0 + k) * q + j0 + j];
-      HIPCHK(hipMemcpyAsync(D.small, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, D.st));
-      dim3 g((unsigned)((D.n + 255) / 256)), b(256);
-      const double *Bk = Bm + (int64_t)k0 * D.n;
-      double *Vj = V + (int64_t)j0 * D.n;
-      if (qq <= 4) hipLaunchKernelGGL(k_project_out<4>, g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
-      else if (qq <= 8) hipLaunchKernelGGL(k_project_out<8>, g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
-      else hipLaunchKernelGGL(k_project_out<16>, g, b, 0, D.st, D.n, Bk, D.n, pp, D.small, qq, Vj, D.n);
+      HIPCHK(hipMemcpyAsync(D.small, w, sizeof(double) * pp * qq, hipMemcpyHostToDevice, D.st));
+      const void *Bk = (const char *)Bm + (int64_t)k0 * D.n * D.es;
+      void *Vj = D.cv(V, j0);
+      if (D.es == 4) launch_project<float>(D, qq, (const float *)Bk, pp, (float *)Vj);
+      else launch_project<double>(D, qq, (const double *)Bk, pp, (double *)Vj);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(D.st));
     }
@@ -314,8 +499,10 @@ static int project_out(Dav &D, const double *Bm, int p, const std::vector<double
   return COSMO_OK;
 }
 
-// AB[:, c0 .. c0+m) = H B[:, c0 .. c0+m)
+// AB[:, c0 .. c0+m) = H B[:, c0 .. c0+m) in the current phase's storage; prec must match
+// it (FP32 storage with the FP32 product, FP64 storage with the FP64 product)
 static int apply_block(Dav &D, int c0, int m, int prec) {
+  if ((prec == COSMO_PREC_FP32) != (D.es == 4)) return fail(COSMO_EINVAL, "apply_block: storage/precision mismatch");
   auto t0 = std::chrono::steady_clock::now();
   for (int j0 = 0; j0 < m; j0 += D.mblock) {
     int mm = std::min(D.mblock, m - j0);
@@ -326,15 +513,19 @@ static int apply_block(Dav &D, int c0, int m, int prec) {
     dim3 g((unsigned)((tot + 255) / 256)), gp((unsigned)((totp + 255) / 256)), b(256);
     int rc;
     if (prec == COSMO_PREC_FP32) {
-      hipLaunchKernelGGL(k_cols_to_rows<float>, gp, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xf, ld);
+      hipLaunchKernelGGL((k_cols_to_rows<float, float>), gp, b, 0, D.st, D.n, mm, (const float *)D.cv(D.Bv, c0 + j0),
+                         D.n, D.Xf, ld);
       rc = op_apply_dev(*D.op, D.Xf, D.Yf, mm, ld, ld, COSMO_PREC_FP32, D.st);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_rows_to_cols<float>, g, b, 0, D.st, D.n, mm, D.Yf, ld, D.col(D.AB, c0 + j0), D.n);
+      hipLaunchKernelGGL((k_rows_to_cols<float, float>), g, b, 0, D.st, D.n, mm, D.Yf, ld,
+                         (float *)D.cv(D.ABv, c0 + j0), D.n);
     } else {
-      hipLaunchKernelGGL(k_cols_to_rows<double>, gp, b, 0, D.st, D.n, mm, D.col(D.B, c0 + j0), D.n, D.Xd, ld);
+      hipLaunchKernelGGL((k_cols_to_rows<double, double>), gp, b, 0, D.st, D.n, mm,
+                         (const double *)D.cv(D.Bv, c0 + j0), D.n, D.Xd, ld);
       rc = op_apply_dev(*D.op, D.Xd, D.Yd, mm, ld, ld, COSMO_PREC_FP64, D.st);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_rows_to_cols<double>, g, b, 0, D.st, D.n, mm, D.Yd, ld, D.col(D.AB, c0 + j0), D.n);
+      hipLaunchKernelGGL((k_rows_to_cols<double, double>), g, b, 0, D.st, D.n, mm, D.Yd, ld,
+                         (double *)D.cv(D.ABv, c0 + j0), D.n);
       D.matvecs64 += mm;
     }
     HIPCHK(hipGetLastError());
@@ -394,44 +585,54 @@ static void sym_eig(int k, const std::vector<double> &A0, std::vector<double> &w
   V.swap(Vr);
 }
 
-// Orthonormalise V (q columns at Vp) against B[:, 0..p) and among themselves, in
-// `rounds` rounds of Gram-Schmidt passes plus Cholesky QR (two for FP64 accuracy; one
-// while H.V itself is FP32, whose vectors are only good to single precision).
+// Orthonormalise V (q columns at Vp, right after B[:, 0..p)) against B and among
+// themselves, in `rounds` rounds of Gram-Schmidt passes plus Cholesky QR (two for FP64
+// accuracy; one while H.V itself is FP32, whose vectors are only good to single
+// precision).  All in
+// the current phase's storage; with FP32 storage the thresholds are those of single
+// precision (overlap 3e-6, a dependent vector keeps < 1e-5 of its norm).
 // Returns the number of kept vectors (compacted to the front); restart=true when the
 // overlap with the basis does not decrease (the reference's restart condition).
-static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &restart, int &kept,
+static int orthonormalise(Dav &D, int p, void *Vp, int q, double ortho, bool &restart, int &kept,
                           int rounds = 2) {
+  const bool f32 = D.es == 4;
+  if (f32) ortho = std::max(ortho, 3e-6);
+  const double drop_rel = f32 ? 1e-5 : 1e-10;
   auto t0 = std::chrono::steady_clock::now();
   restart = false;
   kept = q;
-  std::vector<double> W;
-  if (q > 0) {
-    // unit columns first: the absolute thresholds below then mean what they say even when
-    // the preconditioner has blown a vector up by up to 1e13
-    std::vector<double> G0;
-    int rc0 = gram(D, Vp, q, Vp, q, G0);
-    if (rc0) return rc0;
-    std::vector<double> sc(q);
-    for (int j = 0; j < q; ++j) {
-      double nr = std::sqrt(std::max(G0[(size_t)j * q + j], 0.0));
-      sc[j] = nr > 0 ? 1.0 / nr : 0.0;
-    }
-    HIPCHK(hipMemcpyAsync(D.small, sc.data(), sizeof(double) * q, hipMemcpyHostToDevice, D.st));
-    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((D.n * q + 255) / 256)), dim3(256), 0, D.st, D.n, q, Vp, D.n,
-                       D.small);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(D.st));
-  }
+  // V follows the basis in storage, so one Gram pass gives both B^T V and V^T V; all
+  // thresholds are relative to the column norms (the preconditioner can scale a vector
+  // by up to 1e13), so no separate normalisation pass is needed
+  if (q > 0 && Vp != D.cv(D.Bv, p)) return fail(COSMO_EINVAL, "orthonormalise: vectors must follow the basis");
+  std::vector<double> G, W, Gv;
   for (int round = 0; round < rounds && kept > 0; ++round) {
+    const int k = kept;
     double dprev = 1e7;
+    std::vector<double> base(k, 0.0);  // column norms at the start of the round
     for (int pass = 0; pass < 4; ++pass) {
+      int rc = gram(D, D.Bv, p + k, Vp, k, G);
+      if (rc) return rc;
+      W.assign(G.begin(), G.begin() + (size_t)p * k);
+      Gv.assign(G.begin() + (size_t)p * k, G.end());
+      if (pass == 0)
+        for (int j = 0; j < k; ++j) base[j] = std::sqrt(std::max(Gv[(size_t)j * k + j], 0.0));
       if (!p) break;
-      int rc = gram(D, D.B, p, Vp, kept, W);
-      if (rc) return rc;
       double dcur = 0.0;
-      for (double x : W) dcur = std::max(dcur, std::fabs(x));
-      rc = project_out(D, D.B, p, W, kept, Vp);
+      for (int j = 0; j < k; ++j) {
+        const double nv = std::sqrt(std::max(Gv[(size_t)j * k + j], 0.0));
+        if (nv > 0)
+          for (int a = 0; a < p; ++a) dcur = std::max(dcur, std::fabs(W[(size_t)a * k + j]) / nv);
+      }
+      rc = project_out(D, D.Bv, p, W, k, Vp);
       if (rc) return rc;
+      // V^T V after the projection, B orthonormal: V^T V - W^T W
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+          double s = 0.0;
+          for (int a = 0; a < p; ++a) s += W[(size_t)a * k + i] * W[(size_t)a * k + j];
+          Gv[(size_t)i * k + j] -= s;
+        }
       if (dcur < ortho) break;
       if (dcur > dprev) {
         restart = true;
@@ -440,19 +641,15 @@ static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &
       dprev = dcur;
     }
     if (restart) break;
-    // Cholesky QR in order, dropping vectors whose remaining norm is < 1e-14
-    std::vector<double> G;
-    int rc = gram(D, Vp, kept, Vp, kept, G);
-    if (rc) return rc;
-    const int k = kept;
+    // Cholesky QR of V^T V in order, dropping vectors whose remaining norm is below 1e-14
+    // of their norm at the start of the round, or below drop_rel of the projected norm
     std::vector<double> Rm((size_t)k * k, 0.0);
     std::vector<int> keep;
     for (int j = 0; j < k; ++j) {
       // r_ij = (G_ij - sum_{l<i} r_li r_lj) / r_ii over kept i
-      std::vector<double> rj(k, 0.0);
-      double d = G[j * k + j];
+      double d = Gv[(size_t)j * k + j];
       for (int ii : keep) {
-        double s = G[ii * k + j];
+        double s = Gv[(size_t)ii * k + j];
         for (int l : keep) {
           if (l == ii) break;
           s -= Rm[l * k + ii] * Rm[l * k + j];
@@ -462,8 +659,8 @@ static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &
         d -= s * s;
       }
       double nrm = d > 0 ? std::sqrt(d) : 0.0;
-      double orig = std::sqrt(std::max(G[j * k + j], 0.0));
-      if (nrm < 1e-14 || nrm < 1e-10 * orig) continue;
+      double orig = std::sqrt(std::max(Gv[(size_t)j * k + j], 0.0));
+      if (nrm <= 1e-14 * base[j] || nrm < drop_rel * orig) continue;
       Rm[j * k + j] = nrm;
       keep.push_back(j);
     }
@@ -481,11 +678,21 @@ static int orthonormalise(Dav &D, int p, double *Vp, int q, double ortho, bool &
       }
     }
     // gather kept columns into T scratch, then V <- Vk * Tinv
-    for (int a = 0; a < kk; ++a)
-      HIPCHK(hipMemcpyAsync(D.T + (int64_t)a * D.n, Vp + (int64_t)keep[a] * D.n, sizeof(double) * D.n,
-                            hipMemcpyDeviceToDevice, D.st));
+    for (int a0 = 0; a0 < kk; a0 += 16) {
+      ColList L;
+      const int m = std::min(16, kk - a0);
+      for (int a = 0; a < m; ++a) L.c[a] = keep[a0 + a];
+      dim3 g((unsigned)((D.n + 255) / 256), m);
+      if (D.es == 4)
+        hipLaunchKernelGGL(k_copy_cols<float>, g, dim3(256), 0, D.st, D.n, (const float *)Vp, L,
+                           (float *)D.cv(D.Tv, a0));
+      else
+        hipLaunchKernelGGL(k_copy_cols<double>, g, dim3(256), 0, D.st, D.n, (const double *)Vp, L,
+                           (double *)D.cv(D.Tv, a0));
+      HIPCHK(hipGetLastError());
+    }
     if (kk) {
-      rc = gemm(D, D.T, nullptr, kk, Tinv, nullptr, kk, Vp);
+      int rc = gemm(D, D.Tv, nullptr, kk, Tinv, nullptr, kk, Vp);
       if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(D.st));
@@ -501,12 +708,12 @@ static int collapse(Dav &D, const std::vector<std::vector<double>> &svec, int kp
   std::vector<double> C((size_t)kp * q);
   for (int k = 0; k < kp; ++k)
     for (int j = 0; j < q; ++j) C[(size_t)k * q + j] = k < (int)svec[j].size() ? svec[j][k] : 0.0;
-  int rc = gemm(D, D.B, nullptr, kp, C, nullptr, q, D.T);
+  int rc = gemm(D, D.Bv, nullptr, kp, C, nullptr, q, D.Tv);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(D.B, D.T, sizeof(double) * D.n * q, hipMemcpyDeviceToDevice, D.st));
-  rc = gemm(D, D.AB, nullptr, kp, C, nullptr, q, D.T);
+  HIPCHK(hipMemcpyAsync(D.Bv, D.Tv, (size_t)D.es * D.n * q, hipMemcpyDeviceToDevice, D.st));
+  rc = gemm(D, D.ABv, nullptr, kp, C, nullptr, q, D.Tv);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(D.AB, D.T, sizeof(double) * D.n * q, hipMemcpyDeviceToDevice, D.st));
+  HIPCHK(hipMemcpyAsync(D.ABv, D.Tv, (size_t)D.es * D.n * q, hipMemcpyDeviceToDevice, D.st));
   HIPCHK(hipStreamSynchronize(D.st));
   return COSMO_OK;
 }
@@ -536,7 +743,7 @@ static void diagonalize_S(const std::vector<double> &s, int k, int nume, std::ve
 static int update_S(Dav &D, std::vector<double> &s, int c0, int m) {
   auto t0 = std::chrono::steady_clock::now();
   std::vector<double> W;
-  int rc = gram(D, D.B, c0 + m, D.col(D.AB, c0), m, W);
+  int rc = gram(D, D.Bv, c0 + m, D.cv(D.ABv, c0), m, W);
   if (rc) return rc;
   for (int j = 0; j < m; ++j) {
     int jj = c0 + j;
@@ -597,7 +804,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   D.lim = p.lim;
   D.nume = p.nume;
   D.mblock = std::min(p.mblock, p.nume);
-  int rc = dav_alloc(D);
+  int rc = dav_alloc(D, mixed);
   if (rc) return rc;
   const int nume = p.nume, lim = p.lim;
   double *dD = op->csr.diag;  // device diagonal
@@ -637,6 +844,11 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     niv += added;
   }
   int prec = mixed ? COSMO_PREC_FP32 : COSMO_PREC_FP64;
+  if (mixed) {  // the FP32 phase works on FP32 copies of the basis
+    rc = convert_cols(D, D.B, 8, D.B32, niv);
+    if (rc) return rc;
+    D.use32();
+  }
   rc = apply_block(D, 0, niv, prec);
   if (rc) return rc;
   std::vector<double> s((size_t)lim * (lim + 1) / 2, 0.0);
@@ -655,9 +867,12 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     // keep the whole FP32-grown basis; it is re-orthonormalised in FP64 (vectors added
     // with a single round may have drifted, which matters once the basis nears the whole
     // space), then H.B and S are recomputed in FP64
+    int r = convert_cols(D, D.B32, 4, D.B, kpass);
+    if (r) return r;
+    D.use64();
     bool rs0;
     int kept0 = kpass;
-    int r = orthonormalise(D, 0, D.B, kpass, p.ortho, rs0, kept0, 2);
+    r = orthonormalise(D, 0, D.B, kpass, p.ortho, rs0, kept0, 2);
     if (r) return r;
     kpass = kept0;
     r = apply_block(D, 0, kpass, COSMO_PREC_FP64);
@@ -746,10 +961,10 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
         }
       }
       auto tr = std::chrono::steady_clock::now();
-      rc = gemm(D, D.AB, D.B, kpass, C1, &C2, ncand, D.R);
+      rc = gemm(D, D.ABv, D.Bv, kpass, C1, &C2, ncand, D.Rv);
       if (rc) return rc;
       std::vector<double> G;
-      rc = gram(D, D.R, ncand, D.R, ncand, G);
+      rc = gram(D, D.Rv, ncand, D.Rv, ncand, G);
       if (rc) return rc;
       D.ms_rr += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr).count();
       int nadded = 0, icvc = 0;
@@ -782,23 +997,27 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
         break;
       }
       nncv = nadded;
-      std::vector<double> lam(nadded);
-      for (int a = 0; a < nadded; ++a) {
-        int j = addidx[a];
-        lam[a] = eigs[incv[j] - 1];
-        incv[a] = incv[j];
-        HIPCHK(hipMemcpyAsync(D.col(D.B, kpass + a), D.col(D.R, j), sizeof(double) * n, hipMemcpyDeviceToDevice, D.st));
-      }
-      if (nadded) {
-        HIPCHK(hipMemcpyAsync(D.lam, lam.data(), sizeof(double) * nadded, hipMemcpyHostToDevice, D.st));
-        int64_t tot = n * nadded;
-        hipLaunchKernelGGL(k_precond, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, D.st, n, nadded,
-                           D.col(D.B, kpass), n, D.lam, dD);
+      for (int a0 = 0; a0 < nadded; a0 += 16) {
+        ColList L;
+        const int m = std::min(16, nadded - a0);
+        for (int a = 0; a < m; ++a) {
+          int j = addidx[a0 + a];
+          L.c[a] = j;
+          L.s[a] = eigs[incv[j] - 1];
+        }
+        dim3 g((unsigned)((n + 255) / 256), m);
+        if (D.es == 4)
+          hipLaunchKernelGGL(k_precond_cols<float>, g, dim3(256), 0, D.st, n, (const float *)D.Rv, L,
+                             (float *)D.cv(D.Bv, kpass + a0), dD);
+        else
+          hipLaunchKernelGGL(k_precond_cols<double>, g, dim3(256), 0, D.st, n, (const double *)D.Rv, L,
+                             (double *)D.cv(D.Bv, kpass + a0), dD);
         HIPCHK(hipGetLastError());
       }
+      for (int a = 0; a < nadded; ++a) incv[a] = incv[addidx[a]];
       bool restart = false;
       int kept = nadded;
-      rc = orthonormalise(D, kpass, D.col(D.B, kpass), nadded, p.ortho, restart, kept, fp64_phase ? 2 : 1);
+      rc = orthonormalise(D, kpass, D.cv(D.Bv, kpass), nadded, p.ortho, restart, kept, fp64_phase ? 2 : 1);
       if (rc) return rc;
       nncv = kept;
       if (!restart) first = true;
@@ -840,6 +1059,12 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
       fprintf(stderr, "davidson it %d kpass %d nncv %d e0 %.10f %s\n", nloops, kpass, nncv, eigs[0],
               fp64_phase ? "fp64" : "fp32");
     if (nloops >= p.maxiter) break;
+  }
+  if (D.es == 4) {  // stopped inside the FP32 phase: finish on FP64 copies of its basis
+    rc = convert_cols(D, D.B32, 4, D.B, kpass);
+    if (!rc) rc = convert_cols(D, D.AB32, 4, D.AB, kpass);
+    if (rc) return rc;
+    D.use64();
   }
   if (!omin_to) kpass -= nncv;
   for (int i = 0; i < nume; ++i) old[i] = std::fabs(old[i] - eigs[i]);
