@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "op.h"
@@ -41,7 +42,8 @@ bool dense_symmetric_eigen(int n, const std::vector<double> &A, std::vector<doub
 #define GRAM_SLAB 32    // rows per wave per step: 4 lane groups x 8 rows
 #define GRAM_LDS 33     // LDS column stride of a staged 32-row tile (doubles)
 #define GRAM_BLOCKS 1024
-#define GRAM_SMALL 8192  // doubles in the small device buffer (Gram results, coefficients)
+#define GRAM_SMALL 8192   // doubles in the small device buffer (Gram results, coefficients)
+#define GRAM_UPLOAD 4096  // doubles per pinned upload slot (two 128 x 16 coefficient blocks)
 
 // partial[g][a*q+b] = sum over the rows that block g visits of A[:,a] * Bm[:,b], on the
 // FP64 matrix cores (v_mfma_f64_16x16x4f64: 16 x 16 output tile, K = 4 rows per
@@ -231,6 +233,12 @@ __global__ __launch_bounds__(256) void k_project_out(int64_t n, const V *__restr
     if (j < q) X[(int64_t)j * ldv + i] = (V)((double)X[(int64_t)j * ldv + i] - acc[j]);
 }
 
+// dst[0..count) = src[0..count): coefficients read by one block from the host's pinned
+// buffer through its device mapping (a launch costs the host far less than a copy call)
+__global__ __launch_bounds__(256) void k_fetch(const double *__restrict__ src, double *__restrict__ dst, int count) {
+  for (int t = threadIdx.x; t < count; t += blockDim.x) dst[t] = src[t];
+}
+
 // Up to 16 column indices (and one value per column) passed by value to a launch.
 struct ColList {
   int c[16];
@@ -301,7 +309,11 @@ struct Dav {
   int es = 8;
   void *Bv = nullptr, *ABv = nullptr, *Rv = nullptr, *Tv = nullptr;
   double *partial = nullptr, *segs = nullptr, *small = nullptr;
-  double *hbuf = nullptr;  // pinned host staging of GRAM_SMALL doubles (per host thread, kept)
+  // pinned host staging (per host thread, kept): GRAM_SMALL doubles of Gram read-back,
+  // then two upload slots of GRAM_UPLOAD doubles used alternately
+  double *hbuf = nullptr, *hdev = nullptr;  // hdev: the device's address of hbuf
+  bool up_busy[2] = {false, false};
+  int up_next = 0;
   int G = 0;
   double ms_apply = 0, ms_ortho = 0, ms_rr = 0;
   int matvecs = 0, matvecs64 = 0;
@@ -353,13 +365,51 @@ static int dav_alloc(Dav &D, bool mixed) {
   HIPCHK(dmalloc(&D.segs, sizeof(double) * (size_t)GRAM_SEG * GRAM_MAXP * GRAM_MAXQ));
   HIPCHK(dmalloc(&D.small, sizeof(double) * GRAM_SMALL));
   // the pinned staging buffer belongs to the host thread and is never freed: freeing
-  // pinned memory synchronises the device, which would stall concurrent solves
+  // pinned memory synchronises the device, which would stall concurrent solves.  It is
+  // coherent (fine-grained), so the Gram sums the device writes into it are visible to
+  // the host as soon as the stream has drained.
   static thread_local double *hstage = nullptr;
-  if (!hstage) HIPCHK(hipHostMalloc((void **)&hstage, sizeof(double) * GRAM_SMALL, hipHostMallocDefault));
+  if (!hstage)
+    HIPCHK(hipHostMalloc((void **)&hstage, sizeof(double) * (GRAM_SMALL + 2 * GRAM_UPLOAD),
+                         hipHostMallocMapped | hipHostMallocCoherent));
   D.hbuf = hstage;
+  HIPCHK(hipHostGetDevicePointer((void **)&D.hdev, D.hbuf, 0));
   HIPCHK(hipMemsetAsync(D.B, 0, sizeof(double) * n * D.lim, D.st));
   HIPCHK(hipMemsetAsync(D.AB, 0, sizeof(double) * n * D.lim, D.st));
   D.use64();
+  return COSMO_OK;
+}
+
+// waits for the stream; every pinned upload slot is then free
+static int dsync(Dav &D) {
+  // poll rather than block: the Davidson loop waits on the device several times per
+  // iteration, and a blocking wait adds tens of microseconds of wake-up latency each time
+  hipError_t e;
+  while ((e = hipStreamQuery(D.st)) == hipErrorNotReady) std::this_thread::yield();
+  HIPCHK(e);
+  D.up_busy[0] = D.up_busy[1] = false;
+  return COSMO_OK;
+}
+#define DSYNC(D)                \
+  do {                          \
+    int rc_ = dsync(D);         \
+    if (rc_) return rc_;        \
+  } while (0)
+
+// the pinned slot for the next host->device upload; waits for the stream only if the
+// slot's previous upload may not have run yet (uploads need no synchronisation of their own)
+static int upload_slot(Dav &D, double *&slot) {
+  const int k = D.up_next;
+  D.up_next ^= 1;
+  if (D.up_busy[k]) DSYNC(D);
+  D.up_busy[k] = true;
+  slot = D.hbuf + GRAM_SMALL + (size_t)k * GRAM_UPLOAD;
+  return COSMO_OK;
+}
+// queue the copy of `count` doubles of an upload slot (filled by the host) to D.small
+static int upload(Dav &D, const double *slot, size_t count) {
+  hipLaunchKernelGGL(k_fetch, dim3(1), dim3(256), 0, D.st, D.hdev + (slot - D.hbuf), D.small, (int)count);
+  HIPCHK(hipGetLastError());
   return COSMO_OK;
 }
 
@@ -403,8 +453,7 @@ static int gram(Dav &D, const void *A, int p, const void *Bm, int q, std::vector
   size_t used = 0;
   auto flush = [&]() -> int {
     if (pend.empty()) return COSMO_OK;
-    HIPCHK(hipMemcpyAsync(D.hbuf, D.small, sizeof(double) * used, hipMemcpyDeviceToHost, D.st));
-    HIPCHK(hipStreamSynchronize(D.st));
+    DSYNC(D);  // the sums were written straight into the pinned buffer
     for (const Piece &c : pend)
       for (int a = 0; a < c.pp; ++a)
         for (int b = 0; b < c.qq; ++b) out[(size_t)(c.a0 + a) * q + c.b0 + b] = D.hbuf[c.off + (size_t)a * c.qq + b];
@@ -426,7 +475,7 @@ static int gram(Dav &D, const void *A, int p, const void *Bm, int q, std::vector
       else launch_gram<double>(D, (const double *)Aa, pp, (const double *)Bb, qq);
       hipLaunchKernelGGL(k_reduce_partial, dim3((pq + 255) / 256, GRAM_SEG), dim3(256), 0, D.st, D.G, pq, D.partial,
                          D.segs);
-      hipLaunchKernelGGL(k_reduce_segments, dim3((pq + 255) / 256), dim3(256), 0, D.st, pq, D.segs, D.small + used);
+      hipLaunchKernelGGL(k_reduce_segments, dim3((pq + 255) / 256), dim3(256), 0, D.st, pq, D.segs, D.hdev + used);
       HIPCHK(hipGetLastError());
       pend.push_back({a0, b0, pp, qq, used});
       used += pq;
@@ -451,13 +500,17 @@ static int gemm(Dav &D, const void *A1, const void *A2, int p, const std::vector
     // C1 then C2 chunks, packed in the pinned buffer and sent in one copy
     const int qq = std::min(16, q - j0);
     const size_t pc = (size_t)p * qq;
-    double *c1 = D.hbuf, *c2 = D.hbuf + pc;
+    double *c1;
+    int rc = upload_slot(D, c1);
+    if (rc) return rc;
+    double *c2 = c1 + pc;
     for (int k = 0; k < p; ++k)
       for (int j = 0; j < qq; ++j) c1[k * qq + j] = C1[(size_t)k * q + j0 + j];
     if (C2)
       for (int k = 0; k < p; ++k)
         for (int j = 0; j < qq; ++j) c2[k * qq + j] = (*C2)[(size_t)k * q + j0 + j];
-    HIPCHK(hipMemcpyAsync(D.small, D.hbuf, sizeof(double) * pc * (C2 ? 2 : 1), hipMemcpyHostToDevice, D.st));
+    rc = upload(D, c1, pc * (C2 ? 2 : 1));
+    if (rc) return rc;
     const double *s2 = C2 ? D.small + pc : nullptr;
     void *Yj = D.cv(Y, j0);
     if (D.es == 4)
@@ -465,7 +518,6 @@ static int gemm(Dav &D, const void *A1, const void *A2, int p, const std::vector
     else
       launch_gemm<double>(D, qq, (const double *)A1, C2 ? (const double *)A2 : nullptr, p, s2, (double *)Yj);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(D.st));  // the pinned buffer is refilled by the next chunk
   }
   return COSMO_OK;
 }
@@ -484,16 +536,18 @@ static int project_out(Dav &D, const void *Bm, int p, const std::vector<double> 
     int qq = std::min(16, q - j0);
     for (int k0 = 0; k0 < p; k0 += 128) {
       int pp = std::min(128, p - k0);
-      double *w = D.hbuf;
+      double *w;
+      int rc = upload_slot(D, w);
+      if (rc) return rc;
       for (int k = 0; k < pp; ++k)
         for (int j = 0; j < qq; ++j) w[k * qq + j] = W[(size_t)(k0 + k) * q + j0 + j];
-      HIPCHK(hipMemcpyAsync(D.small, w, sizeof(double) * pp * qq, hipMemcpyHostToDevice, D.st));
+      rc = upload(D, w, (size_t)pp * qq);
+      if (rc) return rc;
       const void *Bk = (const char *)Bm + (int64_t)k0 * D.n * D.es;
       void *Vj = D.cv(V, j0);
       if (D.es == 4) launch_project<float>(D, qq, (const float *)Bk, pp, (float *)Vj);
       else launch_project<double>(D, qq, (const double *)Bk, pp, (double *)Vj);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(D.st));
     }
   }
   return COSMO_OK;
@@ -531,7 +585,6 @@ static int apply_block(Dav &D, int c0, int m, int prec) {
     HIPCHK(hipGetLastError());
     D.matvecs += mm;
   }
-  HIPCHK(hipStreamSynchronize(D.st));
   D.ms_apply += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return COSMO_OK;
 }
@@ -695,7 +748,6 @@ static int orthonormalise(Dav &D, int p, void *Vp, int q, double ortho, bool &re
       int rc = gemm(D, D.Tv, nullptr, kk, Tinv, nullptr, kk, Vp);
       if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(D.st));
     kept = kk;
   }
   D.ms_ortho += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -714,7 +766,7 @@ static int collapse(Dav &D, const std::vector<std::vector<double>> &svec, int kp
   rc = gemm(D, D.ABv, nullptr, kp, C, nullptr, q, D.Tv);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(D.ABv, D.Tv, (size_t)D.es * D.n * q, hipMemcpyDeviceToDevice, D.st));
-  HIPCHK(hipStreamSynchronize(D.st));
+  DSYNC(D);
   return COSMO_OK;
 }
 
@@ -834,7 +886,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     HIPCHK(hipMemsetAsync(D.col(D.B, niv), 0, sizeof(double) * n * (nume - niv), D.st));
     for (int j = niv; j < nume; ++j)
       HIPCHK(hipMemcpyAsync(D.col(D.B, j) + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, D.st));
-    HIPCHK(hipStreamSynchronize(D.st));
+    DSYNC(D);
     int added = nume - niv;
     if (niv > 0) {
       bool rs;
@@ -1114,7 +1166,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   if (evecs)
     for (int i = 0; i < nume; ++i)
       HIPCHK(hipMemcpyAsync(evecs + (int64_t)i * n, D.col(D.B, i), sizeof(double) * n, hipMemcpyDeviceToHost, D.st));
-  HIPCHK(hipStreamSynchronize(D.st));
+  DSYNC(D);
   if (stats) {
     stats->iterations = nloops;
     stats->matvecs = D.matvecs;

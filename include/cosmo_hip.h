@@ -188,6 +188,8 @@ typedef struct {
   int matvecs_fp64;    /* of which in FP64                                    */
   int restarts;
   int ierr;            /* 1024: orthogonalisation failure (fsdavidson.cxx:721) */
+  /* host wall time per stage; work is queued without waiting where nothing is read back,
+     so a stage's time can include device work queued by the stage before it            */
   double ms_total, ms_apply, ms_ortho, ms_rr;
 } cosmo_dav_stats;
 
