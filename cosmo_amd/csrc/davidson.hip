@@ -585,6 +585,7 @@ static int apply_block(Dav &D, int c0, int m, int prec) {
     HIPCHK(hipGetLastError());
     D.matvecs += mm;
   }
+  DSYNC(D);  // keeps the stage times honest (one short wait per block)
   D.ms_apply += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return COSMO_OK;
 }
@@ -750,6 +751,7 @@ static int orthonormalise(Dav &D, int p, void *Vp, int q, double ortho, bool &re
     }
     kept = kk;
   }
+  DSYNC(D);
   D.ms_ortho += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return COSMO_OK;
 }
