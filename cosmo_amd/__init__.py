@@ -97,6 +97,7 @@ SIGNATURES = {
     "cosmo_read_ee": (_I, [_S, _L, _I, _P, _P, _P]),
     "cosmo_batch_create": (_I, [_I, _P, _I, _P]),
     "cosmo_batch_solve": (_I, [_P, _P, _P, _P, _P, _P]),
+    "cosmo_batch_set_single_launch": (_I, [_P, ctypes.c_int64]),
     "cosmo_batch_destroy": (_I, [_P]),
     "cosmo_server_create": (_I, [_P, _P]),
     "cosmo_server_submit": (_I, [_P, _P, _I, _D, _P]),
@@ -355,6 +356,10 @@ class Batch:
         arr = (ctypes.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
         self.h = ctypes.c_void_p()
         check(lib().cosmo_batch_create(len(self.systems), arr, flags, ctypes.byref(self.h)))
+
+    def set_single_launch(self, nmax):
+        """Dimension bound of the one-launch path for small solves (0: off)."""
+        check(lib().cosmo_batch_set_single_launch(self.h, ctypes.c_int64(int(nmax))))
 
     def solve(self, nume, **kw):
         """nume: int or per-system list.  Returns (list of eigenvalue arrays, list of

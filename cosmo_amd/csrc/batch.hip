@@ -1,9 +1,11 @@
 // batch.hip -- many independent solves on one device: a batch call and a serving loop.
 //
-// Every solve owns its operator and Davidson blocks and runs on its own HIP stream from a
-// worker thread, so solves of uneven size overlap on the device; all kernels are
-// deterministic (no atomics in H.X, Gram reductions in fixed order), so a solve gives the
-// same bits alone and in a mix.  The server admits queued requests in arrival order as
+// A batch splits its solves by size.  The small ones (dav_wg.h bounds) run to completion in
+// one kernel launch, one workgroup each, once worker threads have built their operators;
+// the others own their operator and Davidson blocks and run on their own HIP streams from
+// the worker threads, so solves of uneven size overlap on the device.  All kernels are
+// deterministic (no atomics in H.X, reductions in fixed order), so a solve gives the same
+// bits alone and in a mix on either path.  The server admits queued requests in arrival order as
 // soon as a worker is free and the request's exact device footprint (operator from the
 // builder's count pass + Davidson blocks) fits the remaining HBM budget.
 #include <hip/hip_runtime.h>
@@ -18,6 +20,7 @@
 #include <vector>
 
 #include "op.h"
+#include "dav_wg.h"
 
 namespace cosmo {
 int stored_footprint(const System &s, int flags, int64_t *bytes);
@@ -35,27 +38,136 @@ static double now_s(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Davidson parameters of one batched solve of dimension n: a wider basis than the
+// single-solve default keeps small, nearly degenerate spectra from stalling; iteration
+// cap raised accordingly
+static cosmo_dav_params batch_params(const cosmo_dav_params &p0, int nume, int64_t n, double tol) {
+  cosmo_dav_params p = p0;
+  p.nume = (int)std::min<int64_t>(nume, n);
+  if (p.mblock <= 0 || p.mblock > p.nume) p.mblock = p.nume;
+  int lim = std::max(p.lim, p.nume + std::max(20, p.nume));
+  if (lim > n) lim = (int)n;
+  p.lim = lim;
+  p.maxiter = std::max(p.maxiter, 4000);
+  if (tol > 0) p.critr = tol;
+  return p;
+}
+
 // one solve: build, Davidson, destroy (on the calling thread's stream)
 static int solve_one(System *s, int flags, const cosmo_dav_params &p0, int nume, double tol, double *evals,
                      double *resid, cosmo_dav_stats *st) {
   cosmo_op op = nullptr;
   int rc = cosmo_op_create(static_cast<cosmo_system>(s), flags, &op);
   if (rc) return rc;
-  cosmo_dav_params p = p0;
-  const int64_t n = s->dim;
-  p.nume = std::min<int64_t>(nume, n);
-  if (p.mblock <= 0 || p.mblock > p.nume) p.mblock = p.nume;
-  // a wider basis than the single-solve default keeps small, nearly degenerate spectra
-  // from stalling; iteration cap raised accordingly
-  int lim = std::max(p.lim, p.nume + std::max(20, p.nume));
-  if (lim > n) lim = (int)n;
-  p.lim = lim;
-  p.maxiter = std::max(p.maxiter, 4000);
-  if (tol > 0) p.critr = tol;
+  const cosmo_dav_params p = batch_params(p0, nume, s->dim, tol);
   std::vector<double> de(p.nume);
   rc = cosmo_davidson(op, &p, nullptr, evals, de.data(), resid, nullptr, st);
   cosmo_op_destroy(op);
   return rc;
+}
+
+
+// ---- single-launch path ---------------------------------------------------------------------
+// Solves inside the bounds of dav_wg.h, on stored operators already built on the device: one
+// workgroup per solve, all in one launch on the calling thread's stream.
+struct SmallJob {
+  cosmo_op op;
+  cosmo_dav_params p;  // as batch_params gives it
+  double *evals, *resid;
+  cosmo_dav_stats *st;
+  int rc;
+};
+
+static bool fits_single_launch(const cosmo_dav_params &p, int64_t n, int flags, int64_t nmax) {
+  return n <= nmax && n <= DWG_NMAX && p.nume >= 1 && p.nume <= DWG_MB && p.mblock <= DWG_MB && p.lim <= DWG_LIM &&
+         p.nume <= p.lim && !(flags & (COSMO_OP_VIRTUAL | COSMO_OP_FACTORED)) &&
+         !((flags & COSMO_OP_NO_FP32) && p.precision == COSMO_PREC_FP32) && p.ninit == 0;
+}
+
+static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
+  const int ns = (int)jobs.size();
+  if (!ns) return COSMO_OK;
+  hipStream_t st = lib_stream();
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  std::vector<size_t> off(ns + 1, 0);
+  for (int j = 0; j < ns; ++j) {
+    const int64_t n = jobs[j].op->n;
+    off[j + 1] = off[j] + up(sizeof(double) * (size_t)n * (3 * (size_t)jobs[j].p.lim + 2 * DWG_MB));
+  }
+  const size_t obytes = sizeof(double) * 3 * DWG_MB * (size_t)ns, ibytes = sizeof(int) * 6 * (size_t)ns;
+  char *arena = nullptr, *outs = nullptr;
+  DwgSolve *ddesc = nullptr;
+  HIPCHK(dmalloc(&arena, off[ns], st));
+  HIPCHK(dmalloc(&outs, up(obytes) + ibytes, st));
+  HIPCHK(dmalloc(&ddesc, sizeof(DwgSolve) * ns, st));
+  std::vector<DwgSolve> desc(ns);
+  double *dout = (double *)outs;
+  int *dinfo = (int *)(outs + up(obytes));
+  for (int j = 0; j < ns; ++j) {
+    const Op &o = *jobs[j].op;
+    const cosmo_dav_params &p = jobs[j].p;
+    DwgSolve &d = desc[j];
+    const int64_t n = o.n;
+    double *w = (double *)(arena + off[j]);
+    d.n = n;
+    d.rowptr = o.csr.rowptr;
+    d.cols = o.csr.cols;
+    d.v64 = o.csr.v64;
+    d.v32 = o.csr.v32;
+    d.diag = o.csr.diag;
+    d.B = w;
+    d.AB = w + n * p.lim;
+    d.R = w + 2 * n * p.lim;
+    d.X = w + 3 * n * p.lim;
+    d.Y = d.X + n * DWG_MB;
+    d.nume = p.nume;
+    d.lim = p.lim;
+    d.mblock = p.mblock;
+    d.maxiter = p.maxiter > 0 ? p.maxiter : std::max(p.nume * 40, 200);
+    d.critr = p.critr;
+    d.evals = dout + (size_t)j * 3 * DWG_MB;
+    d.dE = d.evals + DWG_MB;
+    d.resid = d.evals + 2 * DWG_MB;
+    d.info = dinfo + 6 * j;
+  }
+  HIPCHK(hipMemcpyAsync(ddesc, desc.data(), sizeof(DwgSolve) * ns, hipMemcpyHostToDevice, st));
+  DwgParams prm;
+  prm.crite = p0.crite;
+  prm.critc = p0.critc;
+  prm.ortho = p0.ortho;
+  prm.switch_tol = p0.mixed_switch > 0 ? p0.mixed_switch : 1e-5;
+  prm.strict = p0.strict;
+  prm.mixed = p0.precision == COSMO_PREC_FP32;
+  auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(k_dav_wg, dim3(ns), dim3(DWG_THREADS), 0, st, (const DwgSolve *)ddesc, prm);
+  HIPCHK(hipGetLastError());
+  std::vector<double> hout(3 * DWG_MB * (size_t)ns);
+  std::vector<int> hinfo(6 * (size_t)ns);
+  HIPCHK(hipMemcpyAsync(hout.data(), dout, obytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hinfo.data(), dinfo, ibytes, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  dfree(arena);
+  dfree(outs);
+  dfree(ddesc);
+  for (int j = 0; j < ns; ++j) {
+    SmallJob &jb = jobs[j];
+    const double *o = hout.data() + (size_t)j * 3 * DWG_MB;
+    const int *inf = hinfo.data() + 6 * j;
+    for (int i = 0; i < jb.p.nume; ++i) {
+      jb.evals[i] = o[i];
+      jb.resid[i] = o[2 * DWG_MB + i];
+    }
+    memset(jb.st, 0, sizeof(*jb.st));
+    jb.st->iterations = inf[0];
+    jb.st->matvecs = inf[1];
+    jb.st->matvecs_fp64 = inf[2];
+    jb.st->restarts = inf[3];
+    jb.st->ierr = inf[4];
+    jb.st->ms_total = ms;
+    jb.rc = inf[5] ? COSMO_OK : COSMO_ECONV;
+  }
+  return COSMO_OK;
 }
 
 }  // namespace cosmo
@@ -66,6 +178,7 @@ struct cosmo_batch_s {
   std::vector<cosmo_system> sys;
   int flags = 0;
   int device = 0;
+  int64_t single_nmax = 2048;  // dimension bound of the single-launch path (0: off)
 };
 
 struct Request {
@@ -165,30 +278,69 @@ int cosmo_batch_solve(cosmo_batch b, const cosmo_dav_params *p, const int *nume_
   for (int i = 0; i < ns; ++i) off[i + 1] = off[i] + nume_per_solve[i];
   std::vector<int> rcs(ns, 0);
   std::vector<cosmo_dav_stats> sts(ns);
-  std::atomic<int> next(0);
-  const int nthreads = std::min(ns, 16);
+  std::vector<std::vector<double>> rbuf(ns);
+  // the solves that fit the single-launch path, and the rest (worker threads, own streams)
+  std::vector<SmallJob> small;
+  std::vector<int> small_idx, large;
+  for (int i = 0; i < ns; ++i) {
+    const int64_t n = cosmo_system_dim(b->sys[i]);
+    const int k = (int)std::min<int64_t>(nume_per_solve[i], n);
+    for (int j = k; j < nume_per_solve[i]; ++j) {  // fewer states than asked: pad
+      evals[off[i] + j] = 0.0;
+      if (resid) resid[off[i] + j] = 0.0;
+    }
+    rbuf[i].assign(std::max(k, 1), 0.0);
+    if (k <= 0) continue;
+    const cosmo_dav_params q = batch_params(*p, k, n, 0.0);
+    if (fits_single_launch(q, n, b->flags, b->single_nmax)) {
+      small.push_back({nullptr, q, evals + off[i], rbuf[i].data(), &sts[i], 0});
+      small_idx.push_back(i);
+    } else {
+      large.push_back(i);
+    }
+  }
+  const int nsmall = (int)small.size(), nlarge = (int)large.size();
+  std::vector<int> brc(nsmall, 0);
+  std::atomic<int> next_s(0), built(0), next_l(0);
   auto t0 = std::chrono::steady_clock::now();
+  // workers first build the small solves' operators, then take the large solves
   auto work = [&]() {
     cosmo_set_device(b->device);
     for (;;) {
-      int i = next.fetch_add(1);
-      if (i >= ns) break;
-      const int64_t n = cosmo_system_dim(b->sys[i]);
-      const int k = (int)std::min<int64_t>(nume_per_solve[i], n);
-      for (int j = k; j < nume_per_solve[i]; ++j) {  // fewer states than asked: pad
-        evals[off[i] + j] = 0.0;
-        if (resid) resid[off[i] + j] = 0.0;
-      }
-      std::vector<double> r(std::max(k, 1));
-      rcs[i] = k > 0 ? solve_one(b->sys[i], b->flags, *p, k, 0.0, evals + off[i], r.data(), &sts[i]) : COSMO_OK;
-      if (resid)
-        for (int j = 0; j < k; ++j) resid[off[i] + j] = r[j];
+      const int j = next_s.fetch_add(1);
+      if (j >= nsmall) break;
+      brc[j] = cosmo_op_create(b->sys[small_idx[j]], b->flags, &small[j].op);
+      if (!brc[j] && hipStreamSynchronize(lib_stream()) != hipSuccess)
+        brc[j] = fail(COSMO_EHIP, "cosmo_batch_solve: operator build");
+      built.fetch_add(1);
+    }
+    for (;;) {
+      const int j = next_l.fetch_add(1);
+      if (j >= nlarge) break;
+      const int i = large[j];
+      const int k = (int)std::min<int64_t>(nume_per_solve[i], cosmo_system_dim(b->sys[i]));
+      rcs[i] = solve_one(b->sys[i], b->flags, *p, k, 0.0, evals + off[i], rbuf[i].data(), &sts[i]);
     }
     release_thread_streams();
   };
+  const int nthreads = std::max(1, std::min(std::max(nsmall, nlarge), 16));
   std::vector<std::thread> th;
   for (int t = 0; t < nthreads; ++t) th.emplace_back(work);
+  while (built.load() < nsmall) std::this_thread::yield();
+  int rc_small = COSMO_OK;
+  for (int j = 0; j < nsmall; ++j)
+    if (brc[j]) rc_small = brc[j];
+  if (!rc_small) rc_small = run_small(small, *p);
   for (auto &t : th) t.join();
+  for (int j = 0; j < nsmall; ++j) {
+    rcs[small_idx[j]] = rc_small ? rc_small : small[j].rc;
+    if (small[j].op) cosmo_op_destroy(small[j].op);
+  }
+  if (resid)
+    for (int i = 0; i < ns; ++i) {
+      const int k = (int)std::min<int64_t>(nume_per_solve[i], cosmo_system_dim(b->sys[i]));
+      for (int j = 0; j < k; ++j) resid[off[i] + j] = rbuf[i][j];
+    }
   if (stats) {
     memset(stats, 0, sizeof(*stats));
     for (auto &s : sts) {
@@ -207,6 +359,12 @@ int cosmo_batch_solve(cosmo_batch b, const cosmo_dav_params *p, const int *nume_
     if (rcs[i] && rcs[i] != COSMO_ECONV) return rcs[i];
   for (int i = 0; i < ns; ++i)
     if (rcs[i]) return fail(COSMO_ECONV, "a batched solve reached maxiter");
+  return COSMO_OK;
+}
+
+int cosmo_batch_set_single_launch(cosmo_batch b, int64_t nmax) {
+  if (!b || nmax < 0) return fail(COSMO_EINVAL, "cosmo_batch_set_single_launch: bad args");
+  b->single_nmax = nmax;
   return COSMO_OK;
 }
 

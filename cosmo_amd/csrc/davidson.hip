@@ -1122,8 +1122,11 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   }
   if (!omin_to) kpass -= nncv;
   for (int i = 0; i < nume; ++i) old[i] = std::fabs(old[i] - eigs[i]);
-  if (kpass != nume) {
-    // refresh the Ritz vectors for the final kpass (S may have grown before the exit)
+  {
+    // refresh the Ritz vectors for the final kpass (S may have grown before the exit).
+    // The reference skips this when kpass == nume (fsdavidson.cxx:755-758), which leaves
+    // the start vectors in place of the Ritz vectors when nume == n; rotating always is
+    // exact when the basis already is the Ritz basis (S diagonal after a collapse).
     diagonalize_S(s, kpass, nume, eigs, svec);
     rc = collapse(D, svec, kpass);
     if (rc) return rc;

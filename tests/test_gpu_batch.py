@@ -31,6 +31,7 @@ def test_batch_equals_alone(gpu):
     systems = [s for s in systems if s.dim > 0]
     nume = [min(5, s.dim) for s in systems]
     b = gpu.Batch(systems)
+    b.set_single_launch(0)  # every solve on the stream path, as cosmo_davidson runs it alone
     evb, rsb, st = b.solve(nume, critr=1e-8)
     assert st["converged"]
     for s, k, e, r in zip(systems, nume, evb, rsb):
@@ -82,3 +83,68 @@ def test_server_results_same_alone_and_in_mix(gpu):
         assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
         assert np.array_equal(r["evals"], ea), "served result differs from the solve alone"
         assert np.max(r["resid"]) <= 1e-6
+
+
+def _dense_lowest(gpu, s, k):
+    w, _ = gpu.Operator.build(s).dense_eigen(vectors=False)
+    return w[:k]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_single_launch_matches_dense(gpu, precision):
+    """Solves small enough for one workgroup each run in one launch; eigenvalues agree with
+    a dense eigensolve and every residual meets critr, in FP64 and mixed precision."""
+    xmls = random_sd_systems(24, 31)
+    systems = [s for s in (gpu.System.from_xml(x) for x in xmls) if 0 < s.dim <= 2500]
+    assert len(systems) >= 10
+    nume = [min(5, s.dim) for s in systems]
+    b = gpu.Batch(systems)
+    b.set_single_launch(1 << 15)
+    ev, rs, st = b.solve(nume, critr=1e-7, precision=precision)
+    assert st["converged"]
+    for s, k, e, r in zip(systems, nume, ev, rs):
+        w = _dense_lowest(gpu, s, k)
+        assert np.max(np.abs(e - w)) < 1e-7, (s.dim, e, w)
+        assert np.max(r) <= 1e-7
+    b.close()
+
+
+def test_single_launch_same_alone_and_in_mix(gpu):
+    """A workgroup's solve does not depend on the other solves of the launch."""
+    xmls = random_sd_systems(16, 47)
+    systems = [s for s in (gpu.System.from_xml(x) for x in xmls) if 0 < s.dim <= 4000]
+    nume = [min(4, s.dim) for s in systems]
+    b = gpu.Batch(systems)
+    b.set_single_launch(1 << 15)
+    ev, rs, _ = b.solve(nume, critr=1e-6, precision=gpu.PREC_FP32)
+    b.close()
+    for i in (0, len(systems) // 2, len(systems) - 1):
+        one = gpu.Batch([systems[i]])
+        one.set_single_launch(1 << 15)
+        e1, r1, _ = one.solve([nume[i]], critr=1e-6, precision=gpu.PREC_FP32)
+        one.close()
+        assert np.array_equal(e1[0], ev[i]) and np.array_equal(r1[0], rs[i])
+
+
+def test_single_launch_tiny_and_full_space(gpu):
+    """Spaces smaller than the requested count (padding) and equal to it (the basis is the
+    whole space from the start): eigenvalues and residuals still exact."""
+    d = tempfile.mkdtemp()
+    cases = [("sd", 1, 1, 6), ("sd", 1, 1, 10), ("sd", 1, 1, 8), ("sd", 1, 2, 9)]
+    systems = []
+    for i, (sh, nv, zv, tj) in enumerate(cases):
+        s = gpu.System.from_xml(synth.write_system(d, sh, "t%d" % i, nv, zv, tj, "+", 100 + i))
+        if s.dim > 0:
+            systems.append(s)
+    assert any(s.dim < 5 for s in systems)
+    b = gpu.Batch(systems)
+    b.set_single_launch(1 << 15)
+    ev, rs, st = b.solve([5] * len(systems), critr=1e-8)
+    assert st["converged"]
+    for s, e, r in zip(systems, ev, rs):
+        k = min(5, s.dim)
+        w = _dense_lowest(gpu, s, k)
+        assert np.max(np.abs(e[:k] - w)) < 1e-9
+        assert np.max(r[:k]) <= 1e-8
+        assert np.all(e[k:] == 0.0)
+    b.close()
