@@ -26,12 +26,14 @@ namespace cosmo {
 int stored_footprint(const System &s, int flags, int64_t *bytes);
 void release_thread_streams();
 
+// device bytes of cosmo_davidson's blocks (dav_alloc in davidson.hip): B, AB, R, T in FP64
+// and FP32, the H.X staging, the Gram partial sums and the small buffers
 int64_t davidson_footprint(int64_t n, int nume, int mblock, int lim) {
   int ldmax = 4;
   while (ldmax < mblock) ldmax <<= 1;
-  const int64_t G = std::min<int64_t>(1024, std::max<int64_t>(1, (n + 31) / 32));
-  return 8 * n * (2 * (int64_t)lim + std::max(nume, mblock) + std::max(lim, 16)) + n * ldmax * (8 + 8 + 4 + 4) +
-         G * 1024 * 8 + 3 * 8192 * 8;
+  const int64_t G = std::min<int64_t>(1024, std::max<int64_t>(1, (n + 127) / 128));
+  const int64_t cols = 2 * (int64_t)lim + std::max(nume, mblock) + std::max(lim, 16);
+  return 12 * n * cols + n * ldmax * (8 + 8 + 4 + 4) + (G + 32) * 2048 * 8 + 8192 * 8 + 256 * 8;
 }
 
 static double now_s(std::chrono::steady_clock::time_point t0) {
