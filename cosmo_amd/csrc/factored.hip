@@ -354,7 +354,12 @@ struct FactOp {
   std::vector<Cls> cls;
   void *Xp = nullptr, *Yp = nullptr, *Xq = nullptr, *Yq = nullptr;
   size_t p_bytes = 0;
-  rocblas_handle blas = nullptr;
+  // The class GEMMs are independent of each other: they are spread over FACT_AUX streams
+  // (one rocBLAS handle each), forked from and joined back into the caller's stream.
+  static constexpr int FACT_AUX = 4;
+  rocblas_handle blas[FACT_AUX] = {};
+  hipStream_t aux[FACT_AUX] = {};
+  hipEvent_t fork = nullptr, join[FACT_AUX] = {};
   int64_t n = 0;
   size_t bytes = 0;
   ~FactOp() {
@@ -363,7 +368,12 @@ struct FactOp {
     dfree(Yp);
     dfree(Xq);
     dfree(Yq);
-    if (blas) rocblas_destroy_handle(blas);
+    for (int j = 0; j < FACT_AUX; ++j) {
+      if (blas[j]) rocblas_destroy_handle(blas[j]);
+      if (join[j]) (void)hipEventDestroy(join[j]);
+      if (aux[j]) (void)hipStreamDestroy(aux[j]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
   }
   template <typename T>
   int put(const std::vector<T> &h, const T **out) {
@@ -645,8 +655,14 @@ int build_factored(Op &op, const System &s) {
       (rc = f.put(step_code, &A.step_code)) || (rc = f.put(hk_pbeg, &A.hk_pbeg)) || (rc = f.put(hk_ptr, &A.hk_ptr)) ||
       (rc = f.put(hk_words, &A.hk_words)) || (rc = f.put(W, &f.W64)) || (rc = f.put(W32, &f.W32)))
     return rc;
-  if (rocblas_create_handle(&f.blas) != rocblas_status_success)
-    return fail(COSMO_EHIP, "factorised operator: rocblas_create_handle failed");
+  HIPCHK(hipEventCreateWithFlags(&f.fork, hipEventDisableTiming));
+  for (int j = 0; j < FactOp::FACT_AUX; ++j) {
+    HIPCHK(hipStreamCreateWithFlags(&f.aux[j], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&f.join[j], hipEventDisableTiming));
+    if (rocblas_create_handle(&f.blas[j]) != rocblas_status_success ||
+        rocblas_set_stream(f.blas[j], f.aux[j]) != rocblas_status_success)
+      return fail(COSMO_EHIP, "factorised operator: rocblas handle setup failed");
+  }
   op.fact = F.release();
   return COSMO_OK;
 }
@@ -730,23 +746,32 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
   hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)A.nblk), dim3(256), 0, st, A.nblk, A.noff, A.poff, A.qoff,
                      A.qnb, X, Xp, Xq, ld);
   HIPCHK(hipGetLastError());
-  if (rocblas_set_stream(f.blas, st) != rocblas_status_success) return fail(COSMO_EHIP, "rocblas_set_stream failed");
+  // fork: the GEMMs start once the gather is done
+  HIPCHK(hipEventRecord(f.fork, st));
+  for (int j = 0; j < FactOp::FACT_AUX; ++j) HIPCHK(hipStreamWaitEvent(f.aux[j], f.fork, 0));
   const T one = 1, zero = 0;
   const auto N_ = rocblas_operation_none;
+  int gi = 0;  // GEMMs dealt round-robin over the auxiliary streams
   for (auto &c : f.cls) {
     const rocblas_int L = (rocblas_int)((int64_t)c.S * ld), N = c.nb, S = c.S;
     const rocblas_int M = (rocblas_int)((int64_t)c.nb * ld);
     T *xc = Xp + c.prow * ld, *yc = Yp + c.prow * ld, *xq = Xq + c.prow * ld, *yq = Yq + c.prow * ld;
+    rocblas_handle h1 = f.blas[gi++ % FactOp::FACT_AUX], h2 = f.blas[gi++ % FactOp::FACT_AUX];
     rocblas_status b1, b2;
     if constexpr (sizeof(T) == 8) {
-      b1 = rocblas_dgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H64, N, &zero, yc, L);
-      b2 = rocblas_dgemm(f.blas, N_, N_, M, S, S, &one, xq, M, c.P64, S, &zero, yq, M);
+      b1 = rocblas_dgemm(h1, N_, N_, L, N, N, &one, xc, L, c.H64, N, &zero, yc, L);
+      b2 = rocblas_dgemm(h2, N_, N_, M, S, S, &one, xq, M, c.P64, S, &zero, yq, M);
     } else {
-      b1 = rocblas_sgemm(f.blas, N_, N_, L, N, N, &one, xc, L, c.H32, N, &zero, yc, L);
-      b2 = rocblas_sgemm(f.blas, N_, N_, M, S, S, &one, xq, M, c.P32, S, &zero, yq, M);
+      b1 = rocblas_sgemm(h1, N_, N_, L, N, N, &one, xc, L, c.H32, N, &zero, yc, L);
+      b2 = rocblas_sgemm(h2, N_, N_, M, S, S, &one, xq, M, c.P32, S, &zero, yq, M);
     }
     if (b1 != rocblas_status_success || b2 != rocblas_status_success)
       return fail(COSMO_EHIP, "factorised apply: rocblas gemm failed");
+  }
+  // join: k_fact reads every class's Yp and Yq
+  for (int j = 0; j < FactOp::FACT_AUX; ++j) {
+    HIPCHK(hipEventRecord(f.join[j], f.aux[j]));
+    HIPCHK(hipStreamWaitEvent(st, f.join[j], 0));
   }
   const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
   dim3 g((unsigned)A.nblk, (unsigned)slices);
