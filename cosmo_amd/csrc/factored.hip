@@ -360,6 +360,7 @@ struct FactOp {
   rocblas_handle blas[FACT_AUX] = {};
   hipStream_t aux[FACT_AUX] = {};
   hipEvent_t fork = nullptr, join[FACT_AUX] = {};
+  std::map<int, int> tuned;  // (element size, ld) -> k_fact slice width (lanes per row)
   int64_t n = 0;
   size_t bytes = 0;
   ~FactOp() {
@@ -703,15 +704,13 @@ static bool launch_fact_r(int rmax, dim3 g, size_t lds, hipStream_t st, const Fa
 
 static inline int align16(size_t b) { return (int)((b + 15) & ~size_t(15)); }
 
+// k_fact launch for one slice width (lpe lanes of 16 bytes per row); 1 = launched,
+// 0 = this width does not fit, <0 = error
 template <typename T>
-static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
-  FactOp &f = *static_cast<FactOp *>(op.fact);
+static int launch_fact(const FactOp &f, int lpe, const T *X, const T *Yp, const T *Yq, T *Y, int m, int ld,
+                       hipStream_t st) {
   constexpr int VPL = Piece16<T>::N;
-  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
-    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
-  // slice width: up to 64 bytes of every row, narrower when the tile would not fit
-  int lpe = std::min(4, ld / VPL);
-  while (lpe > 1 && (size_t)f.maxS * lpe * 16 > 64 * 1024) lpe >>= 1;
+  if (lpe * VPL > ld || (size_t)f.maxS * lpe * 16 > 64 * 1024) return 0;
   const int G = FACT_THREADS / lpe;
   const int rmax = (f.maxS + G - 1) / G;
   const int slices = ld / (lpe * VPL);
@@ -723,8 +722,24 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
   A.lds_hw = A.lds_hptr + align16((size_t)(f.maxS + 1) * 4);
   const size_t lds = (size_t)A.lds_hw + (size_t)f.hopcap * 4;
   A.hop_regs = f.hopcap <= FACT_HOP_REGS * FACT_THREADS;
-  if (lds > 160 * 1024 || f.np2 > 2 * FACT_THREADS)
-    return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
+  if (lds > 160 * 1024 || f.np2 > 2 * FACT_THREADS) return 0;
+  const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
+  dim3 g((unsigned)A.nblk, (unsigned)slices);
+  bool ok;
+  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  if (!ok) return 0;
+  HIPCHK(hipGetLastError());
+  return 1;
+}
+
+template <typename T>
+static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
+  FactOp &f = *static_cast<FactOp *>(op.fact);
+  constexpr int VPL = Piece16<T>::N;
+  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
+    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
   // single-species parts on the class-contiguous copy of X:
   //   Yp_class = Xp_class Hn_class   (neutron-major: S*ld x nb, one column per block)
   //   Yq_class = Xq_class Hp_sector  (proton-major: nb*ld x S, one column per proton row)
@@ -743,8 +758,8 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     f.p_bytes = need;
   }
   T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp, *Xq = (T *)f.Xq, *Yq = (T *)f.Yq;
-  hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)A.nblk), dim3(256), 0, st, A.nblk, A.noff, A.poff, A.qoff,
-                     A.qnb, X, Xp, Xq, ld);
+  hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)f.A.nblk), dim3(256), 0, st, f.A.nblk, f.A.noff, f.A.poff,
+                     f.A.qoff, f.A.qnb, X, Xp, Xq, ld);
   HIPCHK(hipGetLastError());
   // fork: the GEMMs start once the gather is done
   HIPCHK(hipEventRecord(f.fork, st));
@@ -773,14 +788,43 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     HIPCHK(hipEventRecord(f.join[j], f.aux[j]));
     HIPCHK(hipStreamWaitEvent(st, f.join[j], 0));
   }
-  const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
-  dim3 g((unsigned)A.nblk, (unsigned)slices);
-  bool ok;
-  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
-  else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
-  else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
-  if (!ok) return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
-  HIPCHK(hipGetLastError());
+  // Slice width: tuned once per (precision, ld) by timing every width that fits; the
+  // widths give bit-identical results (each output element is summed by one lane in the
+  // same hop order), so the choice only decides speed.
+  const int key = (int)sizeof(T) * 4096 + ld;
+  auto it = f.tuned.find(key);
+  if (it == f.tuned.end()) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int lpe = 4; lpe >= 1; lpe >>= 1) {
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        HIPCHK(hipEventRecord(e0, st));
+        const int r = launch_fact<T>(f, lpe, X, Yp, Yq, Y, m, ld, st);
+        if (r < 0) return r;
+        if (r == 0) break;
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        ms = std::min(ms, t);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = lpe;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!best) return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
+    it = f.tuned.emplace(key, best).first;
+  }
+  const int r = launch_fact<T>(f, it->second, X, Yp, Yq, Y, m, ld, st);
+  if (r < 0) return r;
+  if (r == 0) return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
   return COSMO_OK;
 }
 

@@ -63,6 +63,24 @@ def test_factored_vs_stored(gpu, tmp_path, shell, nv, zv, twojz):
     assert b.bytes() < a.bytes()
 
 
+def test_factored_tuned_width_same_bits(gpu, tmp_path):
+    """The first apply of each (precision, pitch) times every slice width of k_fact and keeps
+    the fastest; every width sums each output in the same order, so the first (tuning) call
+    and the later calls give the same bits, and a second operator of the same system too."""
+    from cosmo_amd import synth
+    xml = synth.write_system(str(tmp_path), "fp", "t", 3, 3, 0, "+", 9)
+    s = gpu.System.from_xml(xml)
+    b1 = gpu.Operator.build(s, gpu.OP_FACTORED)
+    b2 = gpu.Operator.build(s, gpu.OP_FACTORED)
+    X = np.random.default_rng(4).standard_normal((b1.n, 16))
+    for prec in (gpu.PREC_FP64, gpu.PREC_FP32):
+        for m in (3, 8, 16):
+            y0 = b1.apply(X[:, :m], prec)
+            y1 = b1.apply(X[:, :m], prec)
+            y2 = b2.apply(X[:, :m], prec)
+            assert np.array_equal(y0, y1) and np.array_equal(y0, y2)
+
+
 def test_factored_apply_device_padded(gpu, golden):
     """Device-resident vectors with a padded power-of-two pitch; the padding columns hold
     NaN and must neither leak into the result nor be overwritten."""
