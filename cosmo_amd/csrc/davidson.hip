@@ -34,7 +34,7 @@
 #include "op.h"
 
 namespace cosmo {
-bool dense_symmetric_eigen(int n, const std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
+bool ritz_lowest(int k, const std::vector<double> &A0, int want, std::vector<double> &w, std::vector<double> &V);
 
 
 #define GRAM_MAXP 128   // A columns per launch (8 output tiles of 16)
@@ -778,19 +778,23 @@ static void diagonalize_S(const std::vector<double> &s, int k, int nume, std::ve
   std::vector<double> A((size_t)k * k);
   for (int i = 0; i < k; ++i)
     for (int j = 0; j <= i; ++j) A[i * k + j] = A[j * k + i] = s[(size_t)i * (i + 1) / 2 + j];
-  // Householder + implicit QL (O(k^3), k <= 128); cyclic Jacobi only if QL fails
-  std::vector<double> w, V, Vc;
-  if (dense_symmetric_eigen(k, A, w, Vc)) {
-    V.resize((size_t)k * k);
-    for (int e = 0; e < k; ++e)
-      for (int r = 0; r < k; ++r) V[(size_t)e * k + r] = Vc[(size_t)r * k + e];
+  // only the nume lowest pairs are used: Householder + tridiagonal QR eigenvalues + inverse
+  // iteration (ritz.cpp); cyclic Jacobi on the whole matrix if that ever fails
+  const int want = std::min(nume, k);
+  std::vector<double> w, V;
+  svec.assign(nume, std::vector<double>(k, 0.0));
+  if (ritz_lowest(k, A, want, w, V)) {
+    for (int e = 0; e < want; ++e) {
+      eigs[e] = w[e];
+      for (int r = 0; r < k; ++r) svec[e][r] = V[(size_t)e * k + r];
+    }
   } else {
     sym_eig(k, A, w, V);
+    for (int e = 0; e < want; ++e) {
+      eigs[e] = w[e];
+      for (int r = 0; r < k; ++r) svec[e][r] = V[(size_t)e * k + r];
+    }
   }
-  for (int i = 0; i < k; ++i) eigs[i] = w[i];
-  svec.assign(nume, std::vector<double>(k, 0.0));
-  for (int e = 0; e < nume && e < k; ++e)
-    for (int r = 0; r < k; ++r) svec[e][r] = V[(size_t)e * k + r];
 }
 
 // S columns for basis vectors [c0, c0+m): s[j*(j+1)/2 + i] = b_i . ab_j, i <= j

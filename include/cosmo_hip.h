@@ -148,10 +148,10 @@ int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, co
                         int64_t *cols, double *vals);
 /* Write the .HH file of the operator (include/sm/PPMatrix.cxx:413-465 layout). */
 int cosmo_op_write_hh(cosmo_op op, const char *path);
-/* All eigenpairs of a small stored operator (n <= 20000), ascending; evecs n x n
- * column-major (vector j at evecs + j*n) or NULL.  The exact-diagonalisation branch of
- * the reference for n <= limit (include/xsm/DEXHVcsb.cxx:72-108), computed on the host by
- * Householder tridiagonalisation and implicit QL. */
+/* All eigenpairs of a small operator (n <= 20000), ascending; evecs n x n column-major
+ * (vector j at evecs + j*n) or NULL.  The exact-diagonalisation branch of the reference
+ * for n <= limit (include/xsm/DEXHVcsb.cxx:72-108), computed on the GPU: the rows are
+ * scattered into a dense FP64 matrix in HBM and solved by rocSOLVER (dsyevd). */
 int cosmo_op_dense_eigen(cosmo_op op, double *evals, double *evecs);
 /* Last apply/build timings in milliseconds (device events). */
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms);
