@@ -1,0 +1,156 @@
+// jacobi.h -- parallel cyclic Jacobi eigensolver for a small symmetric matrix held by one
+// workgroup in LDS (round-robin pairing, one barrier pair per round), warm-started from the
+// previous eigenvectors.  Used by the single-launch Davidson (dav_wg.h).
+#pragma once
+
+// Eigenpairs of S[0..k) (row stride LIM): eigenvalues ascending in sh.eigs (all k),
+// sh.order[i] = the column of V holding the i-th; V is warm-started from the previous
+// k_old = sh.kv vectors.  SH provides S (array or pointer), LDS arrays A, V (LIM x LIM),
+// W2 (2 LIM x LIM), jc, js (LIM / 2), partner, pairidx, order, eigs (LIM) and the scalars
+// kv, rot, dctl.
+template <int LIM, class SH>
+__device__ void lds_jacobi(SH &sh, int k) {
+  const int tid = threadIdx.x;
+  const int kv = sh.kv;
+  double *A = sh.A, *V = sh.V, *A2 = sh.W2, *V2 = sh.W2 + LIM * LIM;
+  // Vext = [[V_old, 0], [0, I]]  (into V2), T = S Vext (into A2), A = Vext^T T
+  for (int o = tid; o < k * k; o += blockDim.x) {
+    const int r = o / k, c = o % k;
+    V2[r * LIM + c] = (r < kv && c < kv) ? V[r * LIM + c] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int o = tid; o < k * k; o += blockDim.x) {
+    const int r = o / k, c = o % k;
+    double t = 0.0;
+    for (int l = 0; l < k; ++l) t += sh.S[r * LIM + l] * V2[l * LIM + c];
+    A2[r * LIM + c] = t;
+  }
+  __syncthreads();
+  for (int o = tid; o < k * k; o += blockDim.x) {
+    const int r = o / k, c = o % k;
+    double t = 0.0;
+    for (int l = 0; l < k; ++l) t += V2[l * LIM + r] * A2[l * LIM + c];
+    A[r * LIM + c] = t;
+  }
+  __syncthreads();
+  for (int o = tid; o < k * k; o += blockDim.x) V[(o / k) * LIM + o % k] = V2[(o / k) * LIM + o % k];
+  __syncthreads();
+  // symmetrise (the two products round differently in the two triangles)
+  for (int o = tid; o < k * k; o += blockDim.x) {
+    const int r = o / k, c = o % k;
+    if (r < c) {
+      const double m = 0.5 * (A[r * LIM + c] + A[c * LIM + r]);
+      A[r * LIM + c] = m;
+      A[c * LIM + r] = m;
+    }
+  }
+  __syncthreads();
+  // cyclic Jacobi, round-robin pairing: player K-1 meets player r, players (r+i, r-i) mod
+  // K-1 meet for i = 1..K/2-1; a player >= k (odd k) sits the round out.  An element is
+  // rotated away while it exceeds 1e-15 of the matrix scale (the largest |A_ii|): below
+  // that it is rounding noise, which a threshold relative to small diagonal entries would
+  // chase sweep after sweep.
+  if (tid == 0) {
+    double sc = 0.0;
+    for (int i = 0; i < k; ++i) sc = fmax(sc, fabs(A[i * LIM + i]));
+    for (int i = 0; i < k; ++i)
+      for (int j = i + 1; j < k; ++j) sc = fmax(sc, fabs(A[i * LIM + j]));
+    sh.dctl = 1e-15 * sc + 1e-300;
+  }
+  __syncthreads();
+  const double jtol = sh.dctl;
+  const int K = k + (k & 1);
+  double *Ac = A, *Vc = V, *An = A2, *Vn = V2;  // current and next (swapped each round)
+  for (int sweep = 0; sweep < 40 && k > 1; ++sweep) {
+    if (tid == 0) sh.rot = 0;
+    __syncthreads();
+    for (int rr = 0; rr < K - 1; ++rr) {
+      for (int i = tid; i < K / 2; i += blockDim.x) {
+        int p, q;
+        if (i == 0) {
+          p = K - 1;
+          q = rr;
+        } else {
+          p = (rr + i) % (K - 1);
+          q = (rr - i + (K - 1)) % (K - 1);
+        }
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, sn = 0.0;
+        if (q < k) {
+          const double apq = Ac[p * LIM + q], app = Ac[p * LIM + p], aqq = Ac[q * LIM + q];
+          if (fabs(apq) > jtol) {
+            // rotation zeroing A[p][q]: tan = t solves t^2 + 2 t theta - 1 = 0, smaller root
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+            sh.rot = 1;
+          }
+          sh.partner[p] = q;
+          sh.partner[q] = p;
+          sh.pairidx[p] = i;
+          sh.pairidx[q] = i;
+        } else {
+          if (p < k) {
+            sh.partner[p] = -1;
+            sh.pairidx[p] = i;
+          }
+        }
+        sh.jc[i] = c;
+        sh.js[i] = sn;
+      }
+      __syncthreads();
+      // A' = J^T A J, V' = V J with J[p][p] = J[q][q] = c, J[p][q] = s, J[q][p] = -s (p < q)
+      for (int o = tid; o < k * k; o += blockDim.x) {
+        const int x = o / k, y = o % k;
+        const int px = sh.partner[x], py = sh.partner[y];
+        const double cx = sh.jc[sh.pairidx[x]], sx = sh.js[sh.pairidx[x]];
+        const double cy = sh.jc[sh.pairidx[y]], sy = sh.js[sh.pairidx[y]];
+        // column y of A J: (A J)[r][y] = c_y A[r][y] + (y is q ? s : -s) A[r][py]
+        const double fy = (py < 0) ? 0.0 : (py < y ? sy : -sy);
+        const double fx = (px < 0) ? 0.0 : (px < x ? sx : -sx);
+        const double ay = cy * Ac[x * LIM + y] + (py < 0 ? 0.0 : fy * Ac[x * LIM + py]);
+        double v = cx * ay;
+        if (px >= 0) v += fx * (cy * Ac[px * LIM + y] + (py < 0 ? 0.0 : fy * Ac[px * LIM + py]));
+        An[x * LIM + y] = v;
+        Vn[x * LIM + y] = cy * Vc[x * LIM + y] + (py < 0 ? 0.0 : fy * Vc[x * LIM + py]);
+      }
+      __syncthreads();
+      double *t = Ac;
+      Ac = An;
+      An = t;
+      t = Vc;
+      Vc = Vn;
+      Vn = t;
+    }
+    const int rot = sh.rot;
+    __syncthreads();
+    if (!rot) break;
+  }
+  if (Ac != A) {  // an odd number of rounds left the result in the second buffers
+    for (int o = tid; o < k * k; o += blockDim.x) {
+      const int x = o / k, y = o % k;
+      A[x * LIM + y] = Ac[x * LIM + y];
+      V[x * LIM + y] = Vc[x * LIM + y];
+    }
+    __syncthreads();
+  }
+  // ascending order, lower column first on ties: column e goes to its rank
+  for (int e = tid; e < k; e += blockDim.x) {
+    const double ve = A[e * LIM + e];
+    int rank = 0;
+    for (int j = 0; j < k; ++j) {
+      const double vj = A[j * LIM + j];
+      rank += (vj < ve || (vj == ve && j < e)) ? 1 : 0;
+    }
+    sh.order[rank] = e;
+    sh.eigs[rank] = ve;
+  }
+  if (tid == 0) sh.kv = k;
+  __syncthreads();
+}
+
