@@ -710,7 +710,7 @@ template <typename T>
 static int launch_fact(const FactOp &f, int lpe, const T *X, const T *Yp, const T *Yq, T *Y, int m, int ld,
                        hipStream_t st) {
   constexpr int VPL = Piece16<T>::N;
-  if (lpe * VPL > ld || (size_t)f.maxS * lpe * 16 > 64 * 1024) return 0;
+  if (lpe * VPL > ld || (size_t)f.maxS * lpe * 16 > (lpe == 8 ? 96 : 64) * 1024) return 0;
   const int G = FACT_THREADS / lpe;
   const int rmax = (f.maxS + G - 1) / G;
   const int slices = ld / (lpe * VPL);
@@ -726,7 +726,8 @@ static int launch_fact(const FactOp &f, int lpe, const T *X, const T *Yp, const 
   const T *W = sizeof(T) == 8 ? (const T *)f.W64 : (const T *)f.W32;
   dim3 g((unsigned)A.nblk, (unsigned)slices);
   bool ok;
-  if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  if (lpe == 8) ok = launch_fact_r<T, 8>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
+  else if (lpe == 4) ok = launch_fact_r<T, 4>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
   else if (lpe == 2) ok = launch_fact_r<T, 2>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
   else ok = launch_fact_r<T, 1>(rmax, g, lds, st, A, W, X, Yp, Yq, Y, ld, m);
   if (!ok) return 0;
@@ -788,9 +789,9 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     HIPCHK(hipEventRecord(f.join[j], f.aux[j]));
     HIPCHK(hipStreamWaitEvent(st, f.join[j], 0));
   }
-  // Slice width: tuned once per (precision, ld) by timing every width that fits; the
-  // widths give bit-identical results (each output element is summed by one lane in the
-  // same hop order), so the choice only decides speed.
+  // Slice width (1, 2, 4 or 8 lanes of 16 bytes per row): tuned once per (precision, ld)
+  // by timing every width that fits; the widths give bit-identical results (each output
+  // element is summed by one lane in the same hop order), so the choice only decides speed.
   const int key = (int)sizeof(T) * 4096 + ld;
   auto it = f.tuned.find(key);
   if (it == f.tuned.end()) {
@@ -799,7 +800,7 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     HIPCHK(hipEventCreate(&e1));
     int best = 0;
     float best_ms = 1e30f;
-    for (int lpe = 4; lpe >= 1; lpe >>= 1) {
+    for (int lpe = 8; lpe >= 1; lpe >>= 1) {
       float ms = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {
         HIPCHK(hipEventRecord(e0, st));
