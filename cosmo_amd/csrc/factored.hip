@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <vector>
 
@@ -361,9 +362,21 @@ struct FactOp {
   hipStream_t aux[FACT_AUX] = {};
   hipEvent_t fork = nullptr, join[FACT_AUX] = {};
   std::map<int, int> tuned;  // (element size, ld) -> k_fact slice width (lanes per row)
+  // The whole apply (gather, forked GEMMs, k_fact) captured as a HIP graph per (element
+  // size, ld, m, X, Y) once the same call repeats, then replayed by one launch.
+  static constexpr size_t FACT_GRAPHS = 64;
+  using GraphKey = std::tuple<int, int, int, const void *, const void *>;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::map<GraphKey, int> seen;  // calls per key; -1: capture failed, stay eager
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    seen.clear();
+  }
   int64_t n = 0;
   size_t bytes = 0;
   ~FactOp() {
+    drop_graphs();
     for (void *p : owned) dfree(p);
     dfree(Xp);
     dfree(Yp);
@@ -735,29 +748,10 @@ static int launch_fact(const FactOp &f, int lpe, const T *X, const T *Yp, const 
   return 1;
 }
 
+// Everything one apply puts on the stream (no waits, no allocation), up to the join of the
+// forked GEMMs; k_fact follows with the chosen slice width.
 template <typename T>
-static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
-  FactOp &f = *static_cast<FactOp *>(op.fact);
-  constexpr int VPL = Piece16<T>::N;
-  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
-    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
-  // single-species parts on the class-contiguous copy of X:
-  //   Yp_class = Xp_class Hn_class   (neutron-major: S*ld x nb, one column per block)
-  //   Yq_class = Xq_class Hp_sector  (proton-major: nb*ld x S, one column per proton row)
-  const size_t need = (size_t)f.n * ld * sizeof(T);
-  if (f.p_bytes < need) {
-    dfree(f.Xp);
-    dfree(f.Yp);
-    dfree(f.Xq);
-    dfree(f.Yq);
-    f.Xp = f.Yp = f.Xq = f.Yq = nullptr;
-    f.p_bytes = 0;
-    HIPCHK(dmalloc(&f.Xp, need, st));
-    HIPCHK(dmalloc(&f.Yp, need, st));
-    HIPCHK(dmalloc(&f.Xq, need, st));
-    HIPCHK(dmalloc(&f.Yq, need, st));
-    f.p_bytes = need;
-  }
+static int fact_enqueue_single(FactOp &f, const T *X, int ld, hipStream_t st) {
   T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp, *Xq = (T *)f.Xq, *Yq = (T *)f.Yq;
   hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)f.A.nblk), dim3(256), 0, st, f.A.nblk, f.A.noff, f.A.poff,
                      f.A.qoff, f.A.qnb, X, Xp, Xq, ld);
@@ -789,12 +783,76 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     HIPCHK(hipEventRecord(f.join[j], f.aux[j]));
     HIPCHK(hipStreamWaitEvent(st, f.join[j], 0));
   }
+  return COSMO_OK;
+}
+
+// Captures one apply (gather, forked GEMMs, k_fact) on st into an executable graph.
+// False, with the stream out of capture mode and nothing kept, when the capture fails or
+// holds a node other than kernels, event waits and empty joins (a memset node in
+// particular: they replay unreliably here).
+template <typename T>
+static bool fact_capture(FactOp &f, int lpe, const T *X, const T *Yp, const T *Yq, T *Y, int m, int ld,
+                         hipStream_t st, hipGraphExec_t *out) {
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  int r = fact_enqueue_single<T>(f, X, ld, st);
+  if (r == COSMO_OK) r = launch_fact<T>(f, lpe, X, Yp, Yq, Y, m, ld, st) == 1 ? COSMO_OK : COSMO_EHIP;
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(st, &graph);
+  bool ok = r == COSMO_OK && ec == hipSuccess && graph;
+  if (ok) {
+    size_t nn = 0;
+    ok = hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess && nn > 0;
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (ok) ok = hipGraphGetNodes(graph, nodes.data(), &nn) == hipSuccess;
+    for (size_t i = 0; ok && i < nn; ++i) {
+      hipGraphNodeType t;
+      ok = hipGraphNodeGetType(nodes[i], &t) == hipSuccess &&
+           (t == hipGraphNodeTypeKernel || t == hipGraphNodeTypeEmpty || t == hipGraphNodeTypeWaitEvent ||
+            t == hipGraphNodeTypeEventRecord);
+    }
+  }
+  if (ok) ok = hipGraphInstantiate(out, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipGetLastError();
+  return ok;
+}
+
+template <typename T>
+static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
+  FactOp &f = *static_cast<FactOp *>(op.fact);
+  constexpr int VPL = Piece16<T>::N;
+  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
+    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
+  // single-species parts on the class-contiguous copy of X:
+  //   Yp_class = Xp_class Hn_class   (neutron-major: S*ld x nb, one column per block)
+  //   Yq_class = Xq_class Hp_sector  (proton-major: nb*ld x S, one column per proton row)
+  const size_t need = (size_t)f.n * ld * sizeof(T);
+  if (f.p_bytes < need) {
+    dfree(f.Xp);
+    dfree(f.Yp);
+    dfree(f.Xq);
+    dfree(f.Yq);
+    f.Xp = f.Yp = f.Xq = f.Yq = nullptr;
+    f.p_bytes = 0;
+    HIPCHK(dmalloc(&f.Xp, need, st));
+    HIPCHK(dmalloc(&f.Yp, need, st));
+    HIPCHK(dmalloc(&f.Xq, need, st));
+    HIPCHK(dmalloc(&f.Yq, need, st));
+    f.p_bytes = need;
+    f.drop_graphs();  // captured graphs point at the old buffers
+  }
+  const T *Yp = (const T *)f.Yp, *Yq = (const T *)f.Yq;
   // Slice width (1, 2, 4 or 8 lanes of 16 bytes per row): tuned once per (precision, ld)
   // by timing every width that fits; the widths give bit-identical results (each output
   // element is summed by one lane in the same hop order), so the choice only decides speed.
   const int key = (int)sizeof(T) * 4096 + ld;
   auto it = f.tuned.find(key);
   if (it == f.tuned.end()) {
+    const int e = fact_enqueue_single<T>(f, X, ld, st);
+    if (e != COSMO_OK) return e;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -821,9 +879,36 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!best) return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
-    it = f.tuned.emplace(key, best).first;
+    f.tuned.emplace(key, best);
+    // the timed launches all wrote Y; one more with the chosen width leaves the same bits
+    const int r = launch_fact<T>(f, best, X, Yp, Yq, Y, m, ld, st);
+    return r < 0 ? r : COSMO_OK;
   }
-  const int r = launch_fact<T>(f, it->second, X, Yp, Yq, Y, m, ld, st);
+  const int lpe = it->second;
+  // Repeated calls on the same buffers replay one captured graph: one host launch instead
+  // of the gather, the ~2 GEMMs per class and k_fact.  The second call with a key captures;
+  // a capture that fails, or records anything but kernels and event nodes, leaves that key
+  // on the eager path.
+  const FactOp::GraphKey gk{(int)sizeof(T), ld, m, (const void *)X, (const void *)Y};
+  auto g = f.graphs.find(gk);
+  if (g != f.graphs.end()) {
+    HIPCHK(hipGraphLaunch(g->second, st));
+    return COSMO_OK;
+  }
+  int &seen = f.seen[gk];
+  if (seen >= 1 && f.graphs.size() < FactOp::FACT_GRAPHS) {
+    hipGraphExec_t exec = nullptr;
+    if (fact_capture<T>(f, lpe, X, Yp, Yq, Y, m, ld, st, &exec)) {
+      f.graphs.emplace(gk, exec);
+      HIPCHK(hipGraphLaunch(exec, st));
+      return COSMO_OK;
+    }
+    seen = -1;
+  }
+  if (seen >= 0) ++seen;
+  const int e = fact_enqueue_single<T>(f, X, ld, st);
+  if (e != COSMO_OK) return e;
+  const int r = launch_fact<T>(f, lpe, X, Yp, Yq, Y, m, ld, st);
   if (r < 0) return r;
   if (r == 0) return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
   return COSMO_OK;

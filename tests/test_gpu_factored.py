@@ -78,7 +78,42 @@ def test_factored_tuned_width_same_bits(gpu, tmp_path):
             y0 = b1.apply(X[:, :m], prec)
             y1 = b1.apply(X[:, :m], prec)
             y2 = b2.apply(X[:, :m], prec)
+            y3 = b1.apply(X[:, :m], prec)  # captured as a graph
+            y4 = b1.apply(X[:, :m], prec)  # graph replay
             assert np.array_equal(y0, y1) and np.array_equal(y0, y2)
+            assert np.array_equal(y0, y3) and np.array_equal(y0, y4)
+
+
+def test_factored_graph_replay_same_bits(gpu, golden):
+    """Repeated applies on the same device buffers and stream are captured once into a HIP
+    graph (second call) and replayed afterwards.  Each replay reads the current contents of
+    X, so new inputs are copied in by ordinary stream work between calls, with host waits in
+    between; every result must equal, bit for bit, an eager apply of the same input on
+    buffers used only once."""
+    import torch
+    g = golden("sd4p4n_s0")
+    s = gpu.System.from_xml(g.xml)
+    b = gpu.Operator.build(s, gpu.OP_FACTORED)
+    n = b.n
+    st = torch.cuda.Stream()
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    for prec, dt in ((gpu.PREC_FP64, torch.float64), (gpu.PREC_FP32, torch.float32)):
+        inputs = [torch.randn(n, 16, dtype=dt, device="cuda", generator=gen) for _ in range(4)]
+        outs = [torch.empty_like(x) for x in inputs]  # all kept alive: distinct addresses, eager path
+        for x, y in zip(inputs, outs):
+            b.apply_device(x[:, :11], y[:, :11], prec, stream=st)
+        st.synchronize()
+        X = torch.empty(n, 16, dtype=dt, device="cuda")
+        Y = torch.empty(n, 16, dtype=dt, device="cuda")
+        with torch.cuda.stream(st):
+            for it in range(10):
+                i = it % len(inputs)
+                X.copy_(inputs[i])
+                Y.fill_(-1.0)
+                b.apply_device(X[:, :11], Y[:, :11], prec, stream=st)
+                st.synchronize()
+                assert torch.equal(Y[:, :11], outs[i][:, :11]), (prec, it)
+                assert torch.all(Y[:, 11:] == -1.0)
 
 
 def test_factored_apply_device_padded(gpu, golden):
