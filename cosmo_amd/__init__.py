@@ -79,6 +79,7 @@ SIGNATURES = {
     "cosmo_op_create": (_I, [_P, _I, _P]),
     "cosmo_op_create_upper": (_I, [_L, _P, _P, _P, _I, _P]),
     "cosmo_op_read_hh": (_I, [_S, _I, _P]),
+    "cosmo_op_read_csb": (_I, [_S, _I, _P]),
     "cosmo_op_destroy": (_I, [_P]),
     "cosmo_op_size": (_L, [_P]),
     "cosmo_op_nnz": (_I, [_P, _P, _P]),
@@ -89,6 +90,7 @@ SIGNATURES = {
     "cosmo_op_apply_device": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "cosmo_op_upper_rows": (_I, [_P, _L, _L, _P, _P, _P, _P]),
     "cosmo_op_write_hh": (_I, [_P, _S]),
+    "cosmo_op_write_csb": (_I, [_P, _S, _I]),
     "cosmo_op_timings": (_I, [_P, _P, _P]),
     "cosmo_op_dense_eigen": (_I, [_P, _P, _P]),
     "cosmo_dav_defaults": (_I, [_P, _I]),
@@ -236,6 +238,13 @@ class Operator:
         check(lib().cosmo_op_read_hh(path.encode(), flags, ctypes.byref(h)))
         return cls(h)
 
+    @classmethod
+    def read_csb(cls, path, flags=OP_STORED):
+        """Stored operator from a symmetric CSB file (SparseMatrixCSB::Write layout)."""
+        h = ctypes.c_void_p()
+        check(lib().cosmo_op_read_csb(path.encode(), flags, ctypes.byref(h)))
+        return cls(h)
+
     def nnz(self):
         u, f = ctypes.c_int64(), ctypes.c_int64()
         check(lib().cosmo_op_nnz(self.h, ctypes.byref(u), ctypes.byref(f)))
@@ -283,6 +292,10 @@ class Operator:
 
     def write_hh(self, path):
         check(lib().cosmo_op_write_hh(self.h, path.encode()))
+
+    def write_csb(self, path, block=0):
+        """Upper triangle as a CSB file (block 0: uint16(sqrt(n)) + 1 rows per block)."""
+        check(lib().cosmo_op_write_csb(self.h, path.encode(), block))
 
     def dense_eigen(self, vectors=True):
         w = np.zeros(self.n)

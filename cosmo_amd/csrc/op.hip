@@ -387,6 +387,157 @@ int cosmo_op_write_hh(cosmo_op op, const char *path) {
   return COSMO_OK;
 }
 
+// CSB file: the layout of SparseMatrixCSB::Write (include/tav/csbmatrix.cxx:553-576) and
+// CSBBlock::Write (:246-254).  Header [bool symmetric][u32 n][u16 BR][u16 BC][u16 nBR]
+// [u16 nBC], then every block, block-row major: [bool isDiagonal][u32 nnz][nnz u16 col]
+// [nnz u16 row][nnz f64].  Symmetric files hold the upper triangle: block (I, J), J >= I,
+// has the entries of rows [I*BR, (I+1)*BR) and columns [J*BC, (J+1)*BC) with col >= row,
+// in ascending (row, col) order (PPMatrix2CSBBlk, include/sm/PPMatrix.cxx:571-656).
+int cosmo_op_write_csb(cosmo_op op, const char *path, int block) {
+  if (!op || !path || block < 0 || block > 65535) return fail(COSMO_EINVAL, "cosmo_op_write_csb: bad args");
+  if (op->is_virtual) return fail(COSMO_EUNSUP, "virtual operator has no stored rows");
+  const int64_t n = op->n;
+  if (n > (int64_t)0xffffffffLL) return fail(COSMO_EUNSUP, "CSB file: dimension beyond 2^32-1");
+  // default block size as DEXHVcsb picks it (include/xsm/DEXHVcsb.cxx:63)
+  const int64_t br = block ? block : (int64_t)(std::sqrt(1. * (double)n) + 1);
+  if (br > 65535) return fail(COSMO_EUNSUP, "CSB file: block size beyond 65535");
+  const int64_t nb = (n + br - 1) / br;
+  if (nb > 65535) return fail(COSMO_EUNSUP, "CSB file: more than 65535 block rows");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(COSMO_EIO, std::string("cannot write ") + path);
+  const bool sym = true;
+  const uint32_t n32 = (uint32_t)n;
+  const uint16_t b16 = (uint16_t)br, nb16 = (uint16_t)nb;
+  fwrite(&sym, sizeof(bool), 1, f);
+  fwrite(&n32, 4, 1, f);
+  fwrite(&b16, 2, 1, f);
+  fwrite(&b16, 2, 1, f);
+  fwrite(&nb16, 2, 1, f);
+  fwrite(&nb16, 2, 1, f);
+  std::vector<int64_t> cnt, rp, cols, start(nb + 1);
+  std::vector<double> vals, bv;
+  std::vector<uint16_t> bc, brow;
+  for (int64_t I = 0; I < nb; ++I) {
+    const int64_t r0 = I * br, r1 = std::min(n, r0 + br);
+    cnt.assign(r1 - r0, 0);
+    int rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), nullptr, nullptr, nullptr);
+    rp.assign(r1 - r0 + 1, 0);
+    for (int64_t i = 0; i < r1 - r0; ++i) rp[i + 1] = rp[i] + cnt[i];
+    cols.resize(rp.back());
+    vals.resize(rp.back());
+    if (!rc) rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), rp.data(), cols.data(), vals.data());
+    if (rc) {
+      fclose(f);
+      return rc;
+    }
+    // entries of the block row grouped by block column, each group in (row, col) order
+    std::fill(start.begin(), start.end(), 0);
+    for (int64_t e = 0; e < rp.back(); ++e) start[cols[e] / br + 1]++;
+    for (int64_t J = 0; J < nb; ++J) start[J + 1] += start[J];
+    bc.resize(rp.back());
+    brow.resize(rp.back());
+    bv.resize(rp.back());
+    std::vector<int64_t> at(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < r1 - r0; ++i)
+      for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+        const int64_t J = cols[e] / br, q = at[J]++;
+        bc[q] = (uint16_t)(cols[e] - J * br);
+        brow[q] = (uint16_t)i;
+        bv[q] = vals[e];
+      }
+    for (int64_t J = 0; J < nb; ++J) {
+      const bool diag = J == I;
+      const uint32_t nnz = (uint32_t)(start[J + 1] - start[J]);
+      fwrite(&diag, sizeof(bool), 1, f);
+      fwrite(&nnz, 4, 1, f);
+      fwrite(bc.data() + start[J], 2, nnz, f);
+      fwrite(brow.data() + start[J], 2, nnz, f);
+      fwrite(bv.data() + start[J], 8, nnz, f);
+    }
+  }
+  if (fclose(f)) return fail(COSMO_EIO, std::string("write failed: ") + path);
+  return COSMO_OK;
+}
+
+// Symmetric CSB file -> stored operator (SparseMatrixCSB::Read, csbmatrix.cxx:524-552).
+// A diagonal entry the file lacks (dropped below 1e-14 by the builder) reads as 0.
+int cosmo_op_read_csb(const char *path, int flags, cosmo_op *out) {
+  if (!path || !out) return fail(COSMO_EINVAL, "cosmo_op_read_csb: null argument");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(COSMO_EIO, std::string("File ") + path + " is not found");
+  bool sym = false;
+  uint32_t n32 = 0;
+  uint16_t br = 0, bcl = 0, nbr = 0, nbc = 0;
+  auto bad = [&](const char *m) {
+    fclose(f);
+    return fail(COSMO_EIO, std::string("CSB file ") + path + ": " + m);
+  };
+  if (fread(&sym, sizeof(bool), 1, f) != 1 || fread(&n32, 4, 1, f) != 1 || fread(&br, 2, 1, f) != 1 ||
+      fread(&bcl, 2, 1, f) != 1 || fread(&nbr, 2, 1, f) != 1 || fread(&nbc, 2, 1, f) != 1)
+    return bad("truncated header");
+  const int64_t n = n32;
+  if (!sym) {
+    fclose(f);
+    return fail(COSMO_EUNSUP, std::string("CSB file ") + path + ": not symmetric (H is stored as its upper triangle)");
+  }
+  if (br == 0 || bcl == 0 || (int64_t)nbr * br < n || (int64_t)nbc * bcl < n ||
+      (n > 0 && ((int64_t)(nbr - 1) * br >= n || (int64_t)(nbc - 1) * bcl >= n)))
+    return bad("block sizes do not match the dimension");
+  std::vector<int64_t> rp(n + 1, 0), cols;
+  std::vector<double> vals;
+  std::vector<int64_t> R, C;
+  std::vector<double> V;
+  std::vector<uint16_t> bc16, br16;
+  std::vector<double> bv;
+  for (int64_t I = 0; I < nbr; ++I) {
+    R.clear();
+    C.clear();
+    V.clear();
+    for (int64_t J = 0; J < nbc; ++J) {
+      bool isd = false;
+      uint32_t nnz = 0;
+      if (fread(&isd, sizeof(bool), 1, f) != 1 || fread(&nnz, 4, 1, f) != 1) return bad("truncated block header");
+      bc16.resize(nnz);
+      br16.resize(nnz);
+      bv.resize(nnz);
+      if (fread(bc16.data(), 2, nnz, f) != nnz || fread(br16.data(), 2, nnz, f) != nnz ||
+          fread(bv.data(), 8, nnz, f) != nnz)
+        return bad("truncated block");
+      for (uint32_t e = 0; e < nnz; ++e) {
+        const int64_t r = I * br + br16[e], c = J * bcl + bc16[e];
+        if (br16[e] >= br || bc16[e] >= bcl || r >= n || c >= n) return bad("index outside its block");
+        if (c < r) return bad("entry below the diagonal");
+        R.push_back(r);
+        C.push_back(c);
+        V.push_back(bv[e]);
+      }
+    }
+    // rows of this block row in order; blocks came in ascending J, each in (row, col)
+    // order, so a stable grouping by row leaves every row's columns ascending
+    const int64_t r0 = I * br, r1 = std::min<int64_t>(n, r0 + br);
+    std::vector<int64_t> pos(r1 - r0 + 1, 0);
+    for (int64_t r : R) pos[r - r0 + 1]++;
+    for (int64_t i = 0; i < r1 - r0; ++i) pos[i + 1] += pos[i];
+    const int64_t base = (int64_t)cols.size();
+    cols.resize(base + R.size());
+    vals.resize(base + R.size());
+    std::vector<int64_t> at(pos.begin(), pos.end() - 1);
+    for (size_t e = 0; e < R.size(); ++e) {
+      const int64_t q = base + at[R[e] - r0]++;
+      cols[q] = C[e];
+      vals[q] = V[e];
+    }
+    for (int64_t i = 0; i < r1 - r0; ++i) rp[r0 + i + 1] = base + pos[i + 1];
+  }
+  int extra = fgetc(f);
+  fclose(f);
+  if (extra != EOF) return fail(COSMO_EIO, std::string("CSB file ") + path + ": trailing bytes");
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t e = rp[r] + 1; e < rp[r + 1]; ++e)
+      if (cols[e] <= cols[e - 1]) return fail(COSMO_EIO, std::string("CSB file ") + path + ": columns out of order");
+  return cosmo_op_create_upper(n, rp.data(), cols.data(), vals.data(), flags, out);
+}
+
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms) {
   if (!op) return fail(COSMO_EINVAL, "null operator");
   if (build_ms) *build_ms = op->build_ms;

@@ -118,8 +118,13 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out);
  * feeding davidson_vfile.  Entries with |h| < 1e-14 are kept unless dropped by flags. */
 int cosmo_op_create_upper(int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals,
                           int flags, cosmo_op *out);
-/* Read a .HH file straight into an operator (streamed through pinned host memory). */
+/* Read a .HH file into an operator (rows read on the host, then uploaded as by
+ * cosmo_op_create_upper; |h| < 1e-14 kept, as the file holds them). */
 int cosmo_op_read_hh(const char *path, int flags, cosmo_op *out);
+/* Read a symmetric CSB file (SparseMatrixCSB::Read, include/tav/csbmatrix.cxx:524-552;
+ * layout under cosmo_op_write_csb) into a stored operator; flags as cosmo_op_create_upper.
+ * A diagonal entry missing from the file reads as 0.  Non-symmetric files: COSMO_EUNSUP. */
+int cosmo_op_read_csb(const char *path, int flags, cosmo_op *out);
 int cosmo_op_destroy(cosmo_op op);
 /* MatrixBase::Size (include/kdav/matrixbase.h:20). */
 int64_t cosmo_op_size(cosmo_op op);
@@ -148,6 +153,15 @@ int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, co
                         int64_t *cols, double *vals);
 /* Write the .HH file of the operator (include/sm/PPMatrix.cxx:413-465 layout). */
 int cosmo_op_write_hh(cosmo_op op, const char *path);
+/* Write the operator's upper triangle as a CSB file, the layout of SparseMatrixCSB::Write
+ * (include/tav/csbmatrix.cxx:553-576, CSBBlock::Write :246-254): [bool symmetric=1]
+ * [u32 n][u16 BR][u16 BC][u16 nBR][u16 nBC], then all nBR x nBC blocks, block-row major:
+ * [bool isDiagonal][u32 nnz][nnz u16 col % BC][nnz u16 row % BR][nnz f64], entries in
+ * ascending (row, col) order, blocks below the diagonal empty (PPMatrix2CSBBlk,
+ * include/sm/PPMatrix.cxx:571-656).  block = 0 uses uint16(sqrt(n)) + 1 as DEXHVcsb does
+ * (include/xsm/DEXHVcsb.cxx:63).  An operator built with the default 1e-14 filter writes
+ * the reference's file. */
+int cosmo_op_write_csb(cosmo_op op, const char *path, int block);
 /* All eigenpairs of a small operator (n <= 20000), ascending; evecs n x n column-major
  * (vector j at evecs + j*n) or NULL.  The exact-diagonalisation branch of the reference
  * for n <= limit (include/xsm/DEXHVcsb.cxx:72-108), computed on the GPU: the rows are

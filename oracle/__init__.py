@@ -11,6 +11,7 @@ davidson.py follows include/kdav/fsdavidson.cxx.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import os
 import re
 import struct
@@ -307,6 +308,78 @@ def read_hh(path):
         rowptr[r + 1] = rowptr[r] + nz
     return rowptr, np.concatenate(cols) if cols else np.zeros(0, np.int64), \
         np.concatenate(vals) if vals else np.zeros(0)
+
+
+def read_csb(path):
+    """CSB file of SparseMatrixCSB::Write (include/tav/csbmatrix.cxx:553-576, per block
+    CSBBlock::Write :246-254): [bool sym][u32 n][u16 BR][u16 BC][u16 nBR][u16 nBC], then
+    for every block (row-block major, all nBR x nBC blocks) [bool isDiagonal][u32 nnz]
+    [nnz u16 col][nnz u16 row][nnz f64].  Returns (header dict, list of blocks
+    (i, j, isDiagonal, rows, cols, vals)) and the SHA-256 of the file without the values."""
+    data = open(path, "rb").read()
+    sym, n, br, bc, nbr, nbc = struct.unpack_from("<?IHHHH", data, 0)
+    off = 13
+    blocks = []
+    h = hashlib.sha256(data[:13])
+    for i in range(nbr):
+        for j in range(nbc):
+            isd, nnz = struct.unpack_from("<?I", data, off)
+            h.update(data[off:off + 5 + 4 * nnz])
+            off += 5
+            c = np.frombuffer(data, dtype=np.uint16, count=nnz, offset=off)
+            off += 2 * nnz
+            r = np.frombuffer(data, dtype=np.uint16, count=nnz, offset=off)
+            off += 2 * nnz
+            v = np.frombuffer(data, dtype=np.float64, count=nnz, offset=off)
+            off += 8 * nnz
+            blocks.append((i, j, bool(isd), r, c, v))
+    if off != len(data):
+        raise ValueError("CSB file %s: %d trailing bytes" % (path, len(data) - off))
+    return dict(symmetric=bool(sym), n=n, br=br, bc=bc, nbr=nbr, nbc=nbc), blocks, h.hexdigest()
+
+
+def csb_bytes(n, rowptr, cols, vals, br=None):
+    """The CSB file of an upper-row matrix as SparseMatrixCSB::Write lays it out after
+    PPMatrix2CSBBlk (include/sm/PPMatrix.cxx:571-656): BR = BC = uint16(sqrt(n)) + 1
+    (include/xsm/DEXHVcsb.cxx:63); block (I, J) holds rows of block row I and columns of
+    block column J in ascending (row, col) order; every block is written, isDiagonal on
+    I == J (include/tav/csbmatrix.cxx:246-254,553-576)."""
+    rowptr = np.asarray(rowptr, np.int64)
+    cols = np.asarray(cols, np.int64)
+    vals = np.asarray(vals, np.float64)
+    if br is None:
+        br = int(np.sqrt(float(n)) + 1)
+    nb = (n + br - 1) // br
+    out = [struct.pack("<?IHHHH", True, n, br, br, nb, nb)]
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    for i in range(nb):
+        lo, hi = rowptr[i * br], rowptr[min(n, (i + 1) * br)]
+        r, c, v = rows[lo:hi], cols[lo:hi], vals[lo:hi]
+        jb = c // br
+        for j in range(nb):
+            sel = jb == j
+            out.append(struct.pack("<?I", i == j, int(sel.sum())))
+            out.append((c[sel] - j * br).astype("<u2").tobytes())
+            out.append((r[sel] - i * br).astype("<u2").tobytes())
+            out.append(v[sel].astype("<f8").tobytes())
+    return b"".join(out)
+
+
+def csb_to_upper(hdr, blocks):
+    """Upper rows (rowptr, cols, vals) of a symmetric CSB: entries ordered by row, then
+    column (blocks of one block row are stored in ascending block column)."""
+    n, br, bc = hdr["n"], hdr["br"], hdr["bc"]
+    R = [np.asarray(r, np.int64) + i * br for (i, j, _, r, c, v) in blocks]
+    C = [np.asarray(c, np.int64) + j * bc for (i, j, _, r, c, v) in blocks]
+    V = [np.asarray(v) for (_, _, _, _, _, v) in blocks]
+    R = np.concatenate(R) if R else np.zeros(0, np.int64)
+    C = np.concatenate(C) if C else np.zeros(0, np.int64)
+    V = np.concatenate(V) if V else np.zeros(0)
+    o = np.lexsort((C, R))
+    R, C, V = R[o], C[o], V[o]
+    rowptr = np.zeros(n + 1, np.int64)
+    np.add.at(rowptr, R + 1, 1)
+    return np.cumsum(rowptr), C, V
 
 
 def read_ee(path):

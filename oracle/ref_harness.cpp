@@ -8,6 +8,8 @@
 //   csb     <sys.xml> <out.EE> <bloc> <iters> <nstate>
 //           builds SparseMatrixCSB via PPMatrix2CSBBlk, times it, runs the reference
 //           Davidson on it (DavidsonDiagonalization), prints timings as JSON on stdout
+//   csbwrite <sys.xml> <out.csb>
+//           builds CSB and writes it with SparseMatrixCSB::Write
 //   spmm    <sys.xml> <m> <reps> <out.bin>
 //           builds CSB, applies TimesMatrix to m deterministic vectors reps times,
 //           writes n, m, X, Y and prints the mean time per call as JSON
@@ -61,7 +63,7 @@ static void dump_tensor(FILE *f, tav::STensor<spsint, double> &T, uint32_t rank)
 }
 
 int main(int argc, char **argv) {
-  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|spmm <sys.xml> ...\n"); return 2; }
+  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|csbwrite|spmm <sys.xml> ...\n"); return 2; }
   std::string mode = argv[1];
   Setup s;
   load(s, argv[2]);
@@ -84,6 +86,12 @@ int main(int argc, char **argv) {
   uint64_t nnz = 0;
   for (int i = 0; i < FHH.numBlockRow; ++i)
     for (int j = 0; j < FHH.numBlockCol; ++j) nnz += FHH.Blocks[i][j].nnz;
+  if (mode == "csbwrite") {  // the reference's own CSB file writer (csbmatrix.cxx:553-576)
+    FHH.Write(argv[3]);
+    printf("{\"n\": %llu, \"nnz_upper\": %llu, \"block_rows\": %d}\n", (unsigned long long)s.st.n,
+           (unsigned long long)nnz, (int)br);
+    return 0;
+  }
   if (mode == "csb") {
     int bloc = atoi(argv[4]), iters = atoi(argv[5]), nstate = atoi(argv[6]);
     if (bloc < nstate) bloc = nstate;

@@ -4,6 +4,7 @@ Run in a container that has the CoSMo source tree at /root/reference (never on t
 box): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
 into oracle/_ref from those sources, then this script runs them on seeded synthetic
 systems and stores inputs and outputs as small data files (npz / json / text).
+``python make_golden.py csb`` only adds the CSB file keys to the existing JSON files.
 """
 import hashlib
 import json
@@ -68,6 +69,33 @@ def sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
 
+def csb_meta(d, name):
+    """The reference's CSB file of the system in d: its size, SHA-256, and the SHA-256 of
+    everything but the values (header, block flags, counts and indices)."""
+    run([os.path.join(REF, "ref_harness"), "csbwrite", name + ".xml", "ref.csb"], d)
+    path = os.path.join(d, "ref.csb")
+    hdr, blocks, idx_sha = oracle.read_csb(path)
+    return dict(csb_bytes=os.path.getsize(path), csb_sha256=sha(path), csb_index_sha256=idx_sha,
+                csb_block_rows=hdr["br"], csb_num_block_rows=hdr["nbr"])
+
+
+def csb_only():
+    """Adds the CSB file keys to the existing JSON fixtures (the rest is left as is)."""
+    for (name, shell, nv, zv, twojz, par, seed, full, bloc) in CASES:
+        d = tempfile.mkdtemp()
+        try:
+            synth.write_system(d, shell, name, nv, zv, twojz, par, seed)
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            jp = os.path.join(OUT, name + ".json")
+            meta = json.load(open(jp))
+            meta.update(csb_meta(d, name))
+            with open(jp, "w") as f:
+                json.dump(meta, f, indent=1)
+            print(name, meta["csb_bytes"], meta["csb_index_sha256"][:16])
+        finally:
+            shutil.rmtree(d)
+
+
 def main():
     for (name, shell, nv, zv, twojz, par, seed, full, bloc) in CASES:
         d = tempfile.mkdtemp()
@@ -94,6 +122,7 @@ def main():
                         davidson_block=bloc, ref_evals=E.tolist(), ref_dE=dE.tolist(), ref_resid=R.tolist())
             rp, cols, vals = oracle.read_hh(os.path.join(d, name + ".HH"))
             meta["hh_nnz"] = int(rp[-1])
+            meta.update(csb_meta(d, name))
             arrays = dict(esp_keys=ek, esp_vals=ev, vspx_keys=vk, vspx_vals=vv)
             if full:
                 arrays.update(mbs=mbs, hh_rowptr=rp, hh_cols=cols, hh_vals=vals, spmm_x=X, spmm_y=Y)
@@ -115,4 +144,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["csb"]:
+        csb_only()
+    else:
+        main()

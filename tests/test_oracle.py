@@ -183,3 +183,51 @@ def test_known_answer_24mg_usdb(tmp_path):
         if i == 4:
             continue
         assert abs(w[i] - e) < 6e-5, (i, w[i], e)
+
+
+def _filtered_upper(g):
+    """The reference's own .HH rows with the CSB builder's |h| < 1e-14 drop applied
+    (include/sm/PPMatrix.cxx:621)."""
+    n = g.meta["n"]
+    rp, c, v = g.arrays["hh_rowptr"], g.arrays["hh_cols"], g.arrays["hh_vals"]
+    keep = np.abs(v) >= 1e-14
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    rp2 = np.zeros(n + 1, np.int64)
+    np.add.at(rp2, rows[keep] + 1, 1)
+    return np.cumsum(rp2), c[keep], v[keep]
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_oracle_csb_file_is_the_reference_file(golden, name, tmp_path):
+    """The oracle's CSB layout applied to the reference's own rows reproduces, byte for
+    byte, the file SparseMatrixCSB::Write made (its SHA-256 and size); reading it back
+    returns the same rows."""
+    g = golden(name)
+    rp, c, v = _filtered_upper(g)
+    assert rp[-1] == g.meta["nnz_upper_csb"]
+    b = oracle.csb_bytes(g.meta["n"], rp, c, v)
+    assert len(b) == g.meta["csb_bytes"]
+    assert hashlib.sha256(b).hexdigest() == g.meta["csb_sha256"]
+    p = str(tmp_path / "h.csb")
+    open(p, "wb").write(b)
+    hdr, blocks, isha = oracle.read_csb(p)
+    assert isha == g.meta["csb_index_sha256"]
+    assert hdr["br"] == g.meta["csb_block_rows"] and hdr["nbr"] == g.meta["csb_num_block_rows"]
+    rp2, c2, v2 = oracle.csb_to_upper(hdr, blocks)
+    assert np.array_equal(rp2, rp) and np.array_equal(c2, c) and np.array_equal(v2, v)
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_oracle_h_gives_the_reference_csb_structure(golden, name):
+    """The oracle's own H, laid out as CSB, has the reference file's header, block flags,
+    counts and indices (its values are pinned against the reference's .HH rows, to 1e-12)."""
+    g = golden(name)
+    o = oracle.System(g.xml)
+    rp, c, v = o.h_upper(drop_small=True)
+    b = oracle.csb_bytes(g.meta["n"], rp, c, v)
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".csb") as f:
+        f.write(b)
+        f.flush()
+        _, _, isha = oracle.read_csb(f.name)
+    assert isha == g.meta["csb_index_sha256"]
