@@ -895,6 +895,7 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
     HIPCHK(hipGraphLaunch(g->second, st));
     return COSMO_OK;
   }
+  if (f.seen.size() >= 4 * FactOp::FACT_GRAPHS && !f.seen.count(gk)) f.seen.clear();  // bounded bookkeeping
   int &seen = f.seen[gk];
   if (seen >= 1 && f.graphs.size() < FactOp::FACT_GRAPHS) {
     hipGraphExec_t exec = nullptr;

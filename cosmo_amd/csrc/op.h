@@ -33,6 +33,11 @@ struct Op {
 };
 
 int build_stored(Op &op, const System &s, int flags, hipStream_t st);
+// host upper rows -> full symmetric rows on the device (COSMO_EUNSUP: too long for it)
+int upper_to_full_dev(Op &op, int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                      int flags, hipStream_t st);
+// host -> device copy through pinned double buffers
+int h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st);
 int build_factored(Op &op, const System &s);
 void free_factored(Op &op);
 size_t factored_bytes(const Op &op);

@@ -55,9 +55,15 @@ static int ensure_staging(Op &op, size_t elems) {
   return COSMO_OK;
 }
 
-// Upper-triangle rows (cols ascending per row) -> full symmetric rows on the device.
+// Upper-triangle rows (cols ascending per row) -> full symmetric rows on the device.  The
+// GPU assembles them (upper.hip); this host transposition remains for matrices whose
+// columns are too long for its LDS sort.
 static int upload_upper(Op &op, int64_t n, const int64_t *rowptr, const int64_t *cols, const double *vals,
                         int flags, hipStream_t st) {
+  {
+    const int rc = upper_to_full_dev(op, n, rowptr, cols, vals, flags, st);
+    if (rc != COSMO_EUNSUP) return rc;
+  }
   const bool keep = flags & COSMO_OP_KEEP_SMALL;
   std::vector<int64_t> cnt(n + 1, 0);
   for (int64_t r = 0; r < n; ++r)
@@ -193,33 +199,40 @@ int cosmo_op_create_upper(int64_t n, const int64_t *rowptr, const int64_t *cols,
 }
 
 int cosmo_op_read_hh(const char *path, int flags, cosmo_op *out) {
+  if (!path || !out) return fail(COSMO_EINVAL, "cosmo_op_read_hh: null argument");
   FILE *f = fopen(path, "rb");
   if (!f) return fail(COSMO_EIO, std::string("File ") + path + " is not found");
-  uint64_t n = 0;
-  if (fread(&n, 8, 1, f) != 1) {
+  std::vector<char> iobuf((size_t)16 << 20);
+  setvbuf(f, iobuf.data(), _IOFBF, iobuf.size());
+  auto bad = [&](const char *m) {
     fclose(f);
-    return fail(COSMO_EIO, "truncated .HH header");
-  }
-  std::vector<int64_t> rp(n + 1, 0), cols;
-  std::vector<double> vals;
-  std::vector<uint64_t> tmp;
+    return fail(COSMO_EIO, std::string(path) + ": " + m);
+  };
+  if (fseeko(f, 0, SEEK_END) != 0) return bad("cannot size the file");
+  const int64_t fsize = ftello(f);
+  if (fseeko(f, 0, SEEK_SET) != 0) return bad("cannot rewind the file");
+  uint64_t n = 0;
+  if (fsize < 8 || fread(&n, 8, 1, f) != 1) return bad("truncated .HH header");
+  if (n > (uint64_t)(fsize - 8) / 4) return bad("truncated .HH file (rows)");
+  // [u64 n] then per row [u32 nz][nz u64 col][nz f64]: the entry count follows from the size
+  const int64_t rest = fsize - 8 - 4 * (int64_t)n;
+  if (rest % 16) return bad("truncated or malformed .HH file");
+  const int64_t E = rest / 16;
+  std::vector<int64_t> rp(n + 1, 0), cols(E);
+  std::vector<double> vals(E);
+  int64_t off = 0;
   for (uint64_t r = 0; r < n; ++r) {
     uint32_t nz = 0;
-    if (fread(&nz, 4, 1, f) != 1) {
-      fclose(f);
-      return fail(COSMO_EIO, "truncated .HH row header");
-    }
-    tmp.resize(nz);
-    size_t old = vals.size();
-    vals.resize(old + nz);
-    if (fread(tmp.data(), 8, nz, f) != nz || fread(vals.data() + old, 8, nz, f) != nz) {
-      fclose(f);
-      return fail(COSMO_EIO, "truncated .HH row");
-    }
-    for (uint32_t i = 0; i < nz; ++i) cols.push_back((int64_t)tmp[i]);
-    rp[r + 1] = rp[r] + nz;
+    if (fread(&nz, 4, 1, f) != 1) return bad("truncated .HH row header");
+    if (off + (int64_t)nz > E) return bad("row lengths exceed the file");
+    static_assert(sizeof(int64_t) == sizeof(uint64_t), "column words");
+    if (fread(cols.data() + off, 8, nz, f) != nz || fread(vals.data() + off, 8, nz, f) != nz)
+      return bad("truncated .HH row");
+    off += nz;
+    rp[r + 1] = off;
   }
   fclose(f);
+  if (off != E) return fail(COSMO_EIO, std::string(path) + ": trailing bytes after the last row");
   return cosmo_op_create_upper((int64_t)n, rp.data(), cols.data(), vals.data(), flags | COSMO_OP_KEEP_SMALL, out);
 }
 
@@ -325,10 +338,10 @@ int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, co
   HIPCHK(hipStreamSynchronize(st));
   const int64_t e0 = rp[0], e1 = rp.back();
   std::vector<uint32_t> hc(e1 - e0);
-  std::vector<double> hv(e1 - e0);
+  std::vector<double> hv(cols ? e1 - e0 : 0);  // counts alone need no values
   if (e1 > e0) {
     HIPCHK(hipMemcpyAsync(hc.data(), M.cols + e0, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(hv.data(), M.v64 + e0, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
+    if (cols) HIPCHK(hipMemcpyAsync(hv.data(), M.v64 + e0, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   for (int64_t r = r0; r < r1; ++r) {
@@ -350,38 +363,60 @@ int cosmo_op_upper_rows(cosmo_op op, int64_t r0, int64_t r1, int64_t *counts, co
 
 int cosmo_op_write_hh(cosmo_op op, const char *path) {
   if (!op || !path) return fail(COSMO_EINVAL, "null argument");
+  if (op->is_virtual) return fail(COSMO_EUNSUP, "virtual operator has no stored rows");
+  hipSetDevice(op->device);
+  hipStream_t st = lib_stream();
+  const CsrDev &M = op->csr;
   FILE *f = fopen(path, "wb");
   if (!f) return fail(COSMO_EIO, std::string("cannot write ") + path);
+  auto bad = [&](const std::string &m) {
+    fclose(f);
+    return fail(COSMO_EIO, m);
+  };
   uint64_t n = (uint64_t)op->n;
   fwrite(&n, 8, 1, f);
+  // per chunk of rows: one copy of the stored rows, their upper parts (the stored rows
+  // are ascending, so the upper part is the tail from the diagonal on) laid out in one
+  // buffer and written at once
   const int64_t chunk = 1 << 16;
-  std::vector<int64_t> cnt, rp, cols;
-  std::vector<double> vals;
-  std::vector<uint64_t> c64;
+  std::vector<int64_t> rp;
+  std::vector<uint32_t> hc;
+  std::vector<double> hv;
+  std::vector<char> buf;
   for (int64_t r0 = 0; r0 < op->n; r0 += chunk) {
-    int64_t r1 = std::min<int64_t>(op->n, r0 + chunk);
-    cnt.assign(r1 - r0, 0);
-    int rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), nullptr, nullptr, nullptr);
-    if (rc) {
-      fclose(f);
-      return rc;
+    const int64_t r1 = std::min<int64_t>(op->n, r0 + chunk);
+    rp.resize(r1 - r0 + 1);
+    if (hipMemcpyAsync(rp.data(), M.rowptr + r0, sizeof(int64_t) * rp.size(), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return bad("reading the stored rows failed");
+    const int64_t e0 = rp[0], e1 = rp.back();
+    hc.resize(e1 - e0);
+    hv.resize(e1 - e0);
+    if (e1 > e0 &&
+        (hipMemcpyAsync(hc.data(), M.cols + e0, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(hv.data(), M.v64 + e0, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+      return bad("reading the stored rows failed");
+    buf.resize((size_t)(4 * (r1 - r0) + 16 * (e1 - e0)));
+    char *w = buf.data();
+    for (int64_t r = r0; r < r1; ++r) {
+      int64_t a = rp[r - r0] - e0;
+      const int64_t b = rp[r - r0 + 1] - e0;
+      while (a < b && (int64_t)hc[a] < r) ++a;
+      const uint32_t nz = (uint32_t)(b - a);
+      std::memcpy(w, &nz, 4);
+      w += 4;
+      for (int64_t e = a; e < b; ++e) {
+        const uint64_t c = hc[e];
+        std::memcpy(w, &c, 8);
+        w += 8;
+      }
+      std::memcpy(w, hv.data() + a, 8 * (size_t)nz);
+      w += 8 * (size_t)nz;
     }
-    rp.assign(r1 - r0 + 1, 0);
-    for (int64_t i = 0; i < r1 - r0; ++i) rp[i + 1] = rp[i] + cnt[i];
-    cols.resize(rp.back());
-    vals.resize(rp.back());
-    rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), rp.data(), cols.data(), vals.data());
-    if (rc) {
-      fclose(f);
-      return rc;
-    }
-    for (int64_t i = 0; i < r1 - r0; ++i) {
-      uint32_t nz = (uint32_t)cnt[i];
-      fwrite(&nz, 4, 1, f);
-      c64.assign(cols.begin() + rp[i], cols.begin() + rp[i + 1]);
-      fwrite(c64.data(), 8, nz, f);
-      fwrite(vals.data() + rp[i], 8, nz, f);
-    }
+    const size_t len = (size_t)(w - buf.data());
+    if (fwrite(buf.data(), 1, len, f) != len) return bad(std::string("write failed: ") + path);
   }
   if (fclose(f)) return fail(COSMO_EIO, std::string("write failed: ") + path);
   return COSMO_OK;
