@@ -40,14 +40,19 @@ static double now_s(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Davidson parameters of one batched solve of dimension n: a wider basis than the
-// single-solve default keeps small, nearly degenerate spectra from stalling; iteration
+// Basis limit of a batched or served solve: at least nume + 40 up to 45 (the single-launch
+// kernel holds 48).  On the 256-solve sd batch (5 pairs each, FP32 H.V + FP64 refinement)
+// lim 25 / 30 / 36 / 45 / 60 took 2.03 / 1.59 / 1.37 / 1.31 / 1.36 s: the wider basis
+// restarts less often on the slowly converging, nearly degenerate spectra.
+static int batch_lim(int nume) { return std::max(nume + std::max(20, nume), std::min(nume + 40, 45)); }
+
+// Davidson parameters of one batched solve of dimension n: the wider basis above, iteration
 // cap raised accordingly
 static cosmo_dav_params batch_params(const cosmo_dav_params &p0, int nume, int64_t n, double tol) {
   cosmo_dav_params p = p0;
   p.nume = (int)std::min<int64_t>(nume, n);
   if (p.mblock <= 0 || p.mblock > p.nume) p.mblock = p.nume;
-  int lim = std::max(p.lim, p.nume + std::max(20, p.nume));
+  int lim = std::max(p.lim, batch_lim(p.nume));
   if (lim > n) lim = (int)n;
   p.lim = lim;
   p.maxiter = std::max(p.maxiter, 4000);
@@ -397,7 +402,7 @@ int cosmo_server_submit(cosmo_server S, cosmo_system sys, int nume, double tol, 
   int rc = stored_footprint(*sys, 0, &ob);
   if (rc) return rc;
   const int nm = (int)std::min<int64_t>(r.nume, std::max<int64_t>(sys->dim, 1));
-  r.bytes = ob + davidson_footprint(sys->dim, nm, nm, nm + std::max(20, nm));
+  r.bytes = ob + davidson_footprint(sys->dim, nm, nm, (int)std::min<int64_t>(batch_lim(nm), std::max<int64_t>(sys->dim, 1)));
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (r.bytes > S->p.hbm_budget) return fail(COSMO_ENOMEM, "request needs more HBM than the server budget");

@@ -224,8 +224,8 @@ int cosmo_read_ee(const char *path, int64_t n, int maxcount, int *count, double 
 
 /* ---- batched solves ------------------------------------------------------------------ */
 /* Independent systems solved together (DavidsonDiagonalization per system,
- * include/kdav/davidson.cxx:167-223, with a wider basis: lim = nume + max(20, nume), at
- * least 4000 iterations).  Solves of dimension <= the single-launch bound (default 2048,
+ * include/kdav/davidson.cxx:167-223, with a wider basis: lim = the larger of p->lim,
+ * nume + max(20, nume) and min(nume + 40, 45), at most n; at least 4000 iterations).  Solves of dimension <= the single-launch bound (default 2048,
  * at most 32768) with lim <= 48 and nume, block <= 16 on stored operators run to
  * completion in ONE kernel launch, one workgroup per solve; the others run concurrently
  * from worker threads, each on its own stream with the host-driven Davidson.  One

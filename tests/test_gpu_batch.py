@@ -25,6 +25,11 @@ def random_sd_systems(count, seed, max_part=4):
     return out
 
 
+def batch_lim(k, n):
+    """Basis limit the batch and the server give a solve of k pairs and dimension n."""
+    return min(max(k + max(20, k), min(k + 40, 45)), n)
+
+
 def test_batch_equals_alone(gpu):
     xmls = random_sd_systems(12, 5)
     systems = [gpu.System.from_xml(x) for x in xmls]
@@ -36,7 +41,7 @@ def test_batch_equals_alone(gpu):
     assert st["converged"]
     for s, k, e, r in zip(systems, nume, evb, rsb):
         op = gpu.Operator.build(s)
-        ea, _, ra, _, _ = gpu.davidson(op, k, want_vectors=False, critr=1e-8, lim=k + max(20, k), maxiter=4000)
+        ea, _, ra, _, _ = gpu.davidson(op, k, want_vectors=False, critr=1e-8, lim=batch_lim(k, s.dim), maxiter=4000)
         assert np.array_equal(e, ea), "batched result differs from the solve alone"
     # spot-check against dense eigensolves
     for s, k, e in zip(systems, nume, evb):
@@ -64,7 +69,7 @@ def test_server_results_same_alone_and_in_mix(gpu):
     for s in systems:
         k = min(4, s.dim)
         alone.append(gpu.davidson(gpu.Operator.build(s), k, want_vectors=False, critr=1e-6,
-                                  lim=k + max(20, k), maxiter=4000)[0])
+                                  lim=batch_lim(k, s.dim), maxiter=4000)[0])
     # budget that holds only a few solves at once: admission must queue the rest
     foot = [gpu.footprint(s) for s in systems]
     srv = gpu.Server(hbm_budget=int(3 * max(foot) + (64 << 20)), max_active=4, precision=gpu.PREC_FP64)
