@@ -17,6 +17,8 @@ Reported beside it:
     fastest-FP64 over fastest-mixed ratio and solve rates;
   * solve_sd6p6n: sd 6p6n (n = 93,710), block 8, 10 pairs, stored, FP64 against mixed at
     basis limits 30 and 90;
+  * solve_ref16: the reference's reading of block 16 / 10 states (16 pairs targeted, the
+    lowest 10 kept), product form, FP64 against mixed at basis limits 100 and 128;
   * batch: 256 independent sd-shell solves of uneven size (Nv, Zv ~ U{1..6}, 2Jz from
     the allowed range, own seed each), mixed precision, 5 pairs each;
   * serve: a Poisson request stream (sd 70 % / fp 30 %, fp up to 4+4, k ~ U{1..10},
@@ -171,6 +173,8 @@ def main():
     if not args.no_solve:
         out["solve"] = solve_section(ca, fop if fop is not None else op, op, n)
         out["solve_sd6p6n"] = solve_sd6p6n_section(ca, synth)
+        if fop is not None:
+            out["solve_ref16"] = solve_ref16_section(ca, fop)
     del X, Y
     fop = None  # frees the product-form operator
     if not args.no_batch:
@@ -258,6 +262,36 @@ def solve_section(ca, fop, op, n, lims=(60, 100)):
         r["eigenpairs_per_s_mixed"] = round(10.0 / t32, 3)
         res[oname] = r
     return res
+
+
+def solve_ref16_section(ca, fop, lims=(100, 128)):
+    """The reference's reading of "block 16, 10 states": DavidsonDiagonalization with
+    bloc = 16 targets 16 pairs (numemax = mblock = bloc, include/kdav/davidson.cxx:167-223)
+    and saves the lowest nstates = 10.  Product-form operator, FP64 against mixed at two
+    basis limits; fastest against fastest, and the error of the 10 saved eigenvalues."""
+    import numpy as np
+    r, raw = {"targeted_pairs": 16, "saved_pairs": 10, "block": 16}, {}
+    for lim in lims:
+        for label, prec in (("fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
+            t0 = time.perf_counter()
+            ev, de, rs, _, st = ca.davidson(fop, 16, mblock=16, lim=lim, want_vectors=False, critr=1e-6,
+                                            precision=prec)
+            dt = time.perf_counter() - t0
+            raw[(label, lim)] = np.array(ev[:10])
+            r["%s_lim%d" % (label, lim)] = dict(seconds=round(dt, 3), max_resid=float(np.max(rs)),
+                                                iterations=st["iterations"], matvecs=st["matvecs"],
+                                                matvecs_fp64=st["matvecs_fp64"], converged=st["converged"])
+            print("[bench] solve 16-targeted %s lim %d: %.2fs iters %d resid %.2e" % (label, lim, dt,
+                                                                                  st["iterations"], np.max(rs)),
+                  file=sys.stderr, flush=True)
+    best = {lab: min(lims, key=lambda L: r["%s_lim%d" % (lab, L)]["seconds"]) for lab in ("fp64", "mixed")}
+    d = np.abs(raw[("mixed", best["mixed"])] - raw[("fp64", best["fp64"])])
+    r["best_lim"] = best
+    r["abs_dE_MeV"] = [float(x) for x in d]
+    r["max_abs_dE_MeV"] = float(d.max())
+    r["within_1keV"] = bool(d.max() <= 1e-3)
+    r["mixed_speedup"] = round(r["fp64_lim%d" % best["fp64"]]["seconds"] / r["mixed_lim%d" % best["mixed"]]["seconds"], 3)
+    return r
 
 
 def solve_sd6p6n_section(ca, synth, lims=(30, 90)):
