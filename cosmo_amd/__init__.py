@@ -80,6 +80,7 @@ SIGNATURES = {
     "cosmo_op_create_upper": (_I, [_L, _P, _P, _P, _I, _P]),
     "cosmo_op_read_hh": (_I, [_S, _I, _P]),
     "cosmo_op_read_csb": (_I, [_S, _I, _P]),
+    "cosmo_op_create_csb": (_I, [_L, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P]),
     "cosmo_op_destroy": (_I, [_P]),
     "cosmo_op_size": (_L, [_P]),
     "cosmo_op_nnz": (_I, [_P, _P, _P]),
@@ -236,6 +237,28 @@ class Operator:
     def read_hh(cls, path, flags=OP_STORED):
         h = ctypes.c_void_p()
         check(lib().cosmo_op_read_hh(path.encode(), flags, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csb(cls, n, br, bc, nbr, nbc, blocks, flags=OP_STORED):
+        """Stored operator from symmetric CSB blocks: blocks maps (i, j) -> (rows, cols,
+        vals), in-block indices (uint16) and values."""
+        nb = nbr * nbc
+        cnt = np.zeros(nb, np.uint32)
+        keep = []
+        R, C, V = (ctypes.c_void_p * nb)(), (ctypes.c_void_p * nb)(), (ctypes.c_void_p * nb)()
+        for (i, j), (r, c, v) in blocks.items():
+            r = np.ascontiguousarray(r, np.uint16)
+            c = np.ascontiguousarray(c, np.uint16)
+            v = np.ascontiguousarray(v, np.float64)
+            keep += [r, c, v]
+            b = i * nbc + j
+            cnt[b] = len(v)
+            R[b], C[b], V[b] = r.ctypes.data, c.ctypes.data, v.ctypes.data
+        h = ctypes.c_void_p()
+        check(lib().cosmo_op_create_csb(n, br, bc, nbr, nbc, _p(cnt), ctypes.cast(R, ctypes.c_void_p),
+                                        ctypes.cast(C, ctypes.c_void_p), ctypes.cast(V, ctypes.c_void_p), flags,
+                                        ctypes.byref(h)))
         return cls(h)
 
     @classmethod

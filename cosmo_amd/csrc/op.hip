@@ -128,6 +128,74 @@ static int upload_upper(Op &op, int64_t n, const int64_t *rowptr, const int64_t 
   return COSMO_OK;
 }
 
+// Upper rows from the blocks of a symmetric CSB (upper triangle), fetched block-row major
+// in ascending block column by get(I, J, nnz, rows, cols, vals) (0 = ok, else the error to
+// return).  Each row's entries are sorted by column (blocks from PPMatrix2CSBBlk are
+// already in (row, col) order); a repeated column, an index outside its block and an
+// entry below the diagonal are errors.
+template <class Get>
+static int csb_upper_rows(int64_t n, int64_t br, int64_t bc, int64_t nbr, int64_t nbc, Get get,
+                          std::vector<int64_t> &rp, std::vector<int64_t> &cols, std::vector<double> &vals,
+                          std::string &why) {
+  rp.assign(n + 1, 0);
+  cols.clear();
+  vals.clear();
+  std::vector<int64_t> R, C, ord;
+  std::vector<double> V;
+  for (int64_t I = 0; I < nbr; ++I) {
+    R.clear();
+    C.clear();
+    V.clear();
+    for (int64_t J = 0; J < nbc; ++J) {
+      uint32_t nnz = 0;
+      const uint16_t *rr = nullptr, *cc = nullptr;
+      const double *vv = nullptr;
+      const int rc = get(I, J, nnz, rr, cc, vv);
+      if (rc) return rc;
+      for (uint32_t e = 0; e < nnz; ++e) {
+        const int64_t r = I * br + rr[e], c = J * bc + cc[e];
+        if (rr[e] >= br || cc[e] >= bc || r >= n || c >= n) {
+          why = "index outside its block";
+          return COSMO_EINVAL;
+        }
+        if (c < r) {
+          why = "entry below the diagonal";
+          return COSMO_EINVAL;
+        }
+        R.push_back(r);
+        C.push_back(c);
+        V.push_back(vv[e]);
+      }
+    }
+    const int64_t r0 = I * br, r1 = std::min<int64_t>(n, r0 + br);
+    std::vector<int64_t> pos(r1 - r0 + 1, 0);
+    for (int64_t r : R) pos[r - r0 + 1]++;
+    for (int64_t i = 0; i < r1 - r0; ++i) pos[i + 1] += pos[i];
+    ord.resize(R.size());
+    {
+      std::vector<int64_t> at(pos.begin(), pos.end() - 1);
+      for (size_t e = 0; e < R.size(); ++e) ord[at[R[e] - r0]++] = (int64_t)e;
+    }
+    const int64_t base = (int64_t)cols.size();
+    cols.resize(base + R.size());
+    vals.resize(base + R.size());
+    for (int64_t i = 0; i < r1 - r0; ++i) {
+      auto b0 = ord.begin() + pos[i], b1 = ord.begin() + pos[i + 1];
+      std::stable_sort(b0, b1, [&](int64_t x, int64_t y) { return C[x] < C[y]; });
+      for (int64_t k = pos[i]; k < pos[i + 1]; ++k) {
+        cols[base + k] = C[ord[k]];
+        vals[base + k] = V[ord[k]];
+        if (k > pos[i] && cols[base + k] == cols[base + k - 1]) {
+          why = "a column repeated in a row";
+          return COSMO_EINVAL;
+        }
+      }
+      rp[r0 + i + 1] = base + pos[i + 1];
+    }
+  }
+  return COSMO_OK;
+}
+
 }  // namespace cosmo
 
 using namespace cosmo;
@@ -503,7 +571,7 @@ int cosmo_op_read_csb(const char *path, int flags, cosmo_op *out) {
   bool sym = false;
   uint32_t n32 = 0;
   uint16_t br = 0, bcl = 0, nbr = 0, nbc = 0;
-  auto bad = [&](const char *m) {
+  auto bad = [&](const std::string &m) {
     fclose(f);
     return fail(COSMO_EIO, std::string("CSB file ") + path + ": " + m);
   };
@@ -518,59 +586,59 @@ int cosmo_op_read_csb(const char *path, int flags, cosmo_op *out) {
   if (br == 0 || bcl == 0 || (int64_t)nbr * br < n || (int64_t)nbc * bcl < n ||
       (n > 0 && ((int64_t)(nbr - 1) * br >= n || (int64_t)(nbc - 1) * bcl >= n)))
     return bad("block sizes do not match the dimension");
-  std::vector<int64_t> rp(n + 1, 0), cols;
-  std::vector<double> vals;
-  std::vector<int64_t> R, C;
-  std::vector<double> V;
   std::vector<uint16_t> bc16, br16;
   std::vector<double> bv;
-  for (int64_t I = 0; I < nbr; ++I) {
-    R.clear();
-    C.clear();
-    V.clear();
-    for (int64_t J = 0; J < nbc; ++J) {
-      bool isd = false;
-      uint32_t nnz = 0;
-      if (fread(&isd, sizeof(bool), 1, f) != 1 || fread(&nnz, 4, 1, f) != 1) return bad("truncated block header");
-      bc16.resize(nnz);
-      br16.resize(nnz);
-      bv.resize(nnz);
-      if (fread(bc16.data(), 2, nnz, f) != nnz || fread(br16.data(), 2, nnz, f) != nnz ||
-          fread(bv.data(), 8, nnz, f) != nnz)
-        return bad("truncated block");
-      for (uint32_t e = 0; e < nnz; ++e) {
-        const int64_t r = I * br + br16[e], c = J * bcl + bc16[e];
-        if (br16[e] >= br || bc16[e] >= bcl || r >= n || c >= n) return bad("index outside its block");
-        if (c < r) return bad("entry below the diagonal");
-        R.push_back(r);
-        C.push_back(c);
-        V.push_back(bv[e]);
-      }
-    }
-    // rows of this block row in order; blocks came in ascending J, each in (row, col)
-    // order, so a stable grouping by row leaves every row's columns ascending
-    const int64_t r0 = I * br, r1 = std::min<int64_t>(n, r0 + br);
-    std::vector<int64_t> pos(r1 - r0 + 1, 0);
-    for (int64_t r : R) pos[r - r0 + 1]++;
-    for (int64_t i = 0; i < r1 - r0; ++i) pos[i + 1] += pos[i];
-    const int64_t base = (int64_t)cols.size();
-    cols.resize(base + R.size());
-    vals.resize(base + R.size());
-    std::vector<int64_t> at(pos.begin(), pos.end() - 1);
-    for (size_t e = 0; e < R.size(); ++e) {
-      const int64_t q = base + at[R[e] - r0]++;
-      cols[q] = C[e];
-      vals[q] = V[e];
-    }
-    for (int64_t i = 0; i < r1 - r0; ++i) rp[r0 + i + 1] = base + pos[i + 1];
-  }
-  int extra = fgetc(f);
+  auto get = [&](int64_t, int64_t, uint32_t &nnz, const uint16_t *&rr, const uint16_t *&cc,
+                 const double *&vv) -> int {
+    bool isd = false;
+    if (fread(&isd, sizeof(bool), 1, f) != 1 || fread(&nnz, 4, 1, f) != 1) return COSMO_EIO;
+    bc16.resize(nnz);
+    br16.resize(nnz);
+    bv.resize(nnz);
+    if (fread(bc16.data(), 2, nnz, f) != nnz || fread(br16.data(), 2, nnz, f) != nnz ||
+        fread(bv.data(), 8, nnz, f) != nnz)
+      return COSMO_EIO;
+    rr = br16.data();
+    cc = bc16.data();
+    vv = bv.data();
+    return COSMO_OK;
+  };
+  std::vector<int64_t> rp, cols;
+  std::vector<double> vals;
+  std::string why;
+  const int rc = csb_upper_rows(n, br, bcl, nbr, nbc, get, rp, cols, vals, why);
+  if (rc) return bad(why.empty() ? "truncated block" : why);
+  const int extra = fgetc(f);
   fclose(f);
   if (extra != EOF) return fail(COSMO_EIO, std::string("CSB file ") + path + ": trailing bytes");
-  for (int64_t r = 0; r < n; ++r)
-    for (int64_t e = rp[r] + 1; e < rp[r + 1]; ++e)
-      if (cols[e] <= cols[e - 1]) return fail(COSMO_EIO, std::string("CSB file ") + path + ": columns out of order");
   return cosmo_op_create_upper(n, rp.data(), cols.data(), vals.data(), flags, out);
+}
+
+// Symmetric CSB blocks held in memory (SparseMatrixCSB, csbmatrix.cxx:359-377) -> stored
+// operator.  Block (i, j) is entry i * nbc + j of the arrays; upper blocks only.
+int cosmo_op_create_csb(int64_t n, int br, int bc, int nbr, int nbc, const uint32_t *nnz,
+                        const uint16_t *const *rows, const uint16_t *const *cols, const double *const *vals,
+                        int flags, cosmo_op *out) {
+  if (!out || n < 0 || br <= 0 || bc <= 0 || nbr < 0 || nbc < 0 || (nbr && nbc && !nnz))
+    return fail(COSMO_EINVAL, "cosmo_op_create_csb: bad args");
+  if ((int64_t)nbr * br < n || (int64_t)nbc * bc < n)
+    return fail(COSMO_EINVAL, "cosmo_op_create_csb: blocks do not cover the dimension");
+  auto get = [&](int64_t I, int64_t J, uint32_t &k, const uint16_t *&rr, const uint16_t *&cc,
+                 const double *&vv) -> int {
+    const int64_t b = I * nbc + J;
+    k = nnz[b];
+    if (k && (!rows || !cols || !vals || !rows[b] || !cols[b] || !vals[b])) return COSMO_EINVAL;
+    rr = k ? rows[b] : nullptr;
+    cc = k ? cols[b] : nullptr;
+    vv = k ? vals[b] : nullptr;
+    return COSMO_OK;
+  };
+  std::vector<int64_t> rp, cl;
+  std::vector<double> vl;
+  std::string why;
+  const int rc = csb_upper_rows(n, br, bc, nbr, nbc, get, rp, cl, vl, why);
+  if (rc) return fail(rc, "cosmo_op_create_csb: " + (why.empty() ? std::string("missing block arrays") : why));
+  return cosmo_op_create_upper(n, rp.data(), cl.data(), vl.data(), flags, out);
 }
 
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms) {

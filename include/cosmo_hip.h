@@ -125,6 +125,16 @@ int cosmo_op_read_hh(const char *path, int flags, cosmo_op *out);
  * layout under cosmo_op_write_csb) into a stored operator; flags as cosmo_op_create_upper.
  * A diagonal entry missing from the file reads as 0.  Non-symmetric files: COSMO_EUNSUP. */
 int cosmo_op_read_csb(const char *path, int flags, cosmo_op *out);
+/* The same from CSB blocks in memory: the arrays of a symmetric SparseMatrixCSB
+ * (include/tav/csbmatrix.cxx:128-377: BlockRow, BlockCol, numBlockRow, numBlockCol and per
+ * block nnz, indRow, indCol, elem).  Block (i, j) is entry i * nblock_cols + j of nnz, rows,
+ * cols and vals (pointers may be NULL where nnz is 0); only blocks on and above the
+ * diagonal hold entries, each entry with col >= row.  In-block order is free (each row's
+ * entries are sorted by column); a repeated entry is an error.  flags as
+ * cosmo_op_create_upper.  Replaces handing a SparseMatrixCSB to MyMatrixDiagonal. */
+int cosmo_op_create_csb(int64_t n, int block_rows, int block_cols, int nblock_rows, int nblock_cols,
+                        const uint32_t *nnz, const uint16_t *const *rows, const uint16_t *const *cols,
+                        const double *const *vals, int flags, cosmo_op *out);
 int cosmo_op_destroy(cosmo_op op);
 /* MatrixBase::Size (include/kdav/matrixbase.h:20). */
 int64_t cosmo_op_size(cosmo_op op);

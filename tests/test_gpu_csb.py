@@ -37,12 +37,9 @@ def test_write_csb_matches_reference_file(gpu, golden, name, tmp_path):
     assert np.max(np.abs(v - ov)) <= 1e-12 * np.abs(ov).max()
 
 
-@pytest.mark.parametrize("name", FULL_CASES)
-def test_read_reference_csb_file(gpu, golden, name, tmp_path):
-    """The reference's CSB file (rebuilt byte for byte from its own .HH rows and checked
-    against the SHA-256 of SparseMatrixCSB::Write's output) read into an operator: its
-    FP64 product equals the oracle's CSB product of those rows to 1e-12."""
-    g = golden(name)
+def _reference_csb(g, path):
+    """The reference's CSB file rebuilt byte for byte from its own .HH rows (the SHA-256 of
+    SparseMatrixCSB::Write's output is checked); returns the filtered upper rows."""
     n = g.meta["n"]
     rp, c, v = g.arrays["hh_rowptr"], g.arrays["hh_cols"], g.arrays["hh_vals"]
     keep = np.abs(v) >= 1e-14
@@ -52,12 +49,55 @@ def test_read_reference_csb_file(gpu, golden, name, tmp_path):
     frp = np.cumsum(frp)
     b = oracle.csb_bytes(n, frp, c[keep], v[keep])
     assert hashlib.sha256(b).hexdigest() == g.meta["csb_sha256"]
+    open(path, "wb").write(b)
+    return frp, c[keep], v[keep]
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_create_csb_from_blocks(gpu, golden, name, tmp_path):
+    """The reference file's blocks handed over in memory, as a SparseMatrixCSB holds them:
+    the same operator as reading the file, bit for bit; shuffling the entries inside each
+    block changes nothing; a repeated entry is an error."""
+    g = golden(name)
+    n = g.meta["n"]
     p = str(tmp_path / "ref.csb")
-    open(p, "wb").write(b)
+    _reference_csb(g, p)
+    hdr, blocks, _ = oracle.read_csb(p)
+    a = gpu.Operator.read_csb(p)
+    blk = {(i, j): (r, c, v) for (i, j, _, r, c, v) in blocks if len(v)}
+    args = (n, hdr["br"], hdr["bc"], hdr["nbr"], hdr["nbc"])
+    b = gpu.Operator.from_csb(*args, blk)
+    X = np.random.default_rng(2).standard_normal((n, 3))
+    ya = a.apply(X, gpu.PREC_FP64)
+    assert np.array_equal(b.apply(X, gpu.PREC_FP64), ya)
+    rng = np.random.default_rng(9)
+    shuf = {}
+    for key, (r, c, v) in blk.items():
+        o = rng.permutation(len(v))
+        shuf[key] = (r[o], c[o], v[o])
+    b2 = gpu.Operator.from_csb(*args, shuf)
+    assert np.array_equal(b2.apply(X, gpu.PREC_FP64), ya)
+    key = next(iter(blk))
+    r, c, v = blk[key]
+    dup = dict(blk)
+    dup[key] = (np.concatenate([r, r[:1]]), np.concatenate([c, c[:1]]), np.concatenate([v, v[:1]]))
+    with pytest.raises(gpu.CosmoError):
+        gpu.Operator.from_csb(*args, dup)
+
+
+@pytest.mark.parametrize("name", FULL_CASES)
+def test_read_reference_csb_file(gpu, golden, name, tmp_path):
+    """The reference's CSB file (rebuilt byte for byte from its own .HH rows and checked
+    against the SHA-256 of SparseMatrixCSB::Write's output) read into an operator: its
+    FP64 product equals the oracle's CSB product of those rows to 1e-12."""
+    g = golden(name)
+    n = g.meta["n"]
+    p = str(tmp_path / "ref.csb")
+    frp, fc, fv = _reference_csb(g, p)
     op = gpu.Operator.read_csb(p)
     assert op.n == n and op.nnz()[0] == g.meta["nnz_upper_csb"]
     X = np.random.default_rng(3).standard_normal((n, 5))
-    ref = oracle.csb_times(n, frp, c[keep], v[keep], X)
+    ref = oracle.csb_times(n, frp, fc, fv, X)
     y = op.apply(X, gpu.PREC_FP64)
     assert np.max(np.abs(y - ref)) <= 1e-12 * np.abs(ref).max()
 
