@@ -28,7 +28,7 @@ Reported beside it:
     a row sample of the same matrix, on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-batch]
-                       [--no-serve] [--no-cpu] [--calibrate-copy]
+                       [--no-serve] [--no-cpu] [--calibrate-copy] [--profile-solve fp64|mixed]
 """
 import argparse
 import json
@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline row sample (0: n/8)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_spmm_vec_fp4p4n_k16_traffic.json"),
                     help="HBM bytes per launch of the stored kernel from rocprofv3 counters")
+    ap.add_argument("--profile-solve", choices=["fp64", "mixed"], default=None,
+                    help="only one 10-pair product-form solve (block 16, basis limit 100) in this precision, "
+                         "after one warm-up solve; for kernel profiles")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 4 GiB device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -106,6 +109,20 @@ def main():
     tmp = tempfile.mkdtemp(prefix="cosmo_bench_")
     name = "%s%dp%dn" % (args.shell, args.zv, args.nv)
     xml = synth.write_system(tmp, args.shell, name, args.nv, args.zv, 0, "+", args.seed)
+    if args.profile_solve:
+        sysm = ca.System.from_xml(xml)
+        fop = ca.Operator.build(sysm, ca.OP_FACTORED)
+        prec = ca.PREC_FP64 if args.profile_solve == "fp64" else ca.PREC_FP32
+        kw = dict(mblock=16, lim=100, want_vectors=False, critr=1e-6, precision=prec)
+        ca.davidson(fop, 10, **kw)  # warm-up: tuning and captured graphs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev, de, rs, _, st = ca.davidson(fop, 10, **kw)
+        dt = time.perf_counter() - t0
+        print(json.dumps({"solve": args.profile_solve, "seconds": round(dt, 3), "iterations": st["iterations"],
+                          "matvecs": st["matvecs"], "matvecs_fp64": st["matvecs_fp64"],
+                          "max_resid": float(max(rs)), "evals": [round(float(x), 8) for x in ev]}))
+        return
     t0 = time.perf_counter()
     sysm = ca.System.from_xml(xml)
     t_setup = time.perf_counter() - t0
