@@ -24,8 +24,10 @@ Reported beside it:
   * serve: a Poisson request stream (sd 70 % / fp 30 %, fp up to 4+4, k ~ U{1..10},
     tolerance log-uniform in [1e-6, 1e-4]) through the admission-controlled server,
     p50/p99 latency, solves/s and each result's difference to an FP64 solve alone;
-  * cpu_baseline: the CSB product restated in C with OpenMP (FP64, as the reference) on
-    a row sample of the same matrix, on this host's cores.
+  * cpu_baseline: the reference's own CSB product (SparseMatrixCSB::Read + TimesMatrix,
+    OpenMP, FP64) on the whole matrix on this host's cores, run from oracle/_ref when that
+    build is present; cpu_baseline_port: the CSB product restated in C with OpenMP on a
+    row sample of the same matrix.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-batch]
                        [--no-serve] [--no-cpu] [--calibrate-copy] [--profile-solve fp64|mixed]
@@ -199,7 +201,10 @@ def main():
     if not args.no_serve:
         out["serve"] = serve_section(ca, synth)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_section(ca, op, n, k, args.cpu_rows)
+        port = cpu_section(ca, op, n, k, args.cpu_rows)
+        ref = cpu_reference_section(ca, op, n, k)
+        out["cpu_baseline"] = ref if ref is not None else port
+        out["cpu_baseline_port"] = port
     print(json.dumps(out), flush=True)
 
 
@@ -478,6 +483,45 @@ def cpu_section(ca, op, n, k, rows):
     return {"value": round(b / dt / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": "rows 0..%d (%d upper entries) of the same H, FP64 CSB product, k=%d, %d reps, %.3f s/apply"
                       % (R, nnz, k, reps, dt)}
+
+
+def cpu_reference_section(ca, op, n, k, reps=3):
+    """The reference's own CSB product on the host: the whole H written as its CSB file
+    (the layout of SparseMatrixCSB::Write), read back by the reference's
+    SparseMatrixCSB::Read and multiplied by its TimesMatrix (OpenMP over block rows), in
+    oracle/_ref/ref_harness, compiled from the CoSMo sources.  None when that binary is
+    absent."""
+    import json as _json
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.access(exe, os.X_OK):
+        return None
+    d = tempfile.mkdtemp(prefix="cosmo_csb_")
+    path = os.path.join(d, "h.csb")
+    try:
+        t0 = time.perf_counter()
+        op.write_csb(path)
+        t_write = time.perf_counter() - t0
+        print("[bench] cpu reference: CSB file written (%.1f GB) in %.1fs" % (os.path.getsize(path) / 1e9, t_write),
+              file=sys.stderr, flush=True)
+        r = subprocess.run([exe, "csbtimes", path, str(k), str(reps)], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            print("[bench] cpu reference failed: %s" % r.stderr[-300:], file=sys.stderr, flush=True)
+            return None
+        info = _json.loads([ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1])
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+        os.rmdir(d)
+    nnz, dt = int(info["nnz_upper"]), float(info["t_spmm"])
+    b = nnz * (8 + 2 + 2) + n * k * (8 + 8)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print("[bench] cpu reference: TimesMatrix k=%d %.3f s/apply (read %.1fs)" % (k, dt, info["t_read"]),
+          file=sys.stderr, flush=True)
+    return {"value": round(b / dt / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "reference",
+            "sample": "the whole H (%d upper entries, FP64) as its CSB file, read by the reference's "
+                      "SparseMatrixCSB::Read and multiplied by its TimesMatrix, k=%d, %d reps after one warm-up, "
+                      "%.3f s/apply" % (nnz, k, info["reps"], dt)}
 
 
 if __name__ == "__main__":

@@ -10,6 +10,8 @@
 //           Davidson on it (DavidsonDiagonalization), prints timings as JSON on stdout
 //   csbwrite <sys.xml> <out.csb>
 //           builds CSB and writes it with SparseMatrixCSB::Write
+//   csbtimes <file.csb> <m> <reps>
+//           reads a CSB file with SparseMatrixCSB::Read and times TimesMatrix on m vectors
 //   spmm    <sys.xml> <m> <reps> <out.bin>
 //           builds CSB, applies TimesMatrix to m deterministic vectors reps times,
 //           writes n, m, X, Y and prints the mean time per call as JSON
@@ -62,9 +64,33 @@ static void dump_tensor(FILE *f, tav::STensor<spsint, double> &T, uint32_t rank)
   }
 }
 
+// csbtimes <file.csb> <m> <reps>: the reference's SparseMatrixCSB::Read of a CSB file,
+// then its TimesMatrix (csbmatrix.cxx:623-675) on m deterministic vectors, timed
+static int csbtimes(const char *path, int m, int reps) {
+  tav::SparseMatrixCSB<double, uint32_t, uint16_t> FHH(1, 1, true);
+  double t0 = now_s();
+  FHH.Read(path);
+  const double t_read = now_s() - t0;
+  const uint64_t n = FHH.n;
+  uint64_t nnz = 0;
+  for (int i = 0; i < FHH.numBlockRow; ++i)
+    for (int j = 0; j < FHH.numBlockCol; ++j) nnz += FHH.Blocks[i][j].nnz;
+  std::vector<double> X(n * m), Y(n * m);
+  for (int v = 0; v < m; ++v)
+    for (uint64_t i = 0; i < n; ++i) X[v * n + i] = std::sin(0.37 * double(i) + 1.3 * double(v));
+  FHH.TimesMatrix(Y.data(), X.data(), m);  // warm-up
+  t0 = now_s();
+  for (int r = 0; r < reps; ++r) FHH.TimesMatrix(Y.data(), X.data(), m);
+  const double t_mv = (now_s() - t0) / reps;
+  printf("{\"n\": %llu, \"nnz_upper\": %llu, \"t_read\": %.6f, \"t_spmm\": %.6f, \"m\": %d, \"reps\": %d}\n",
+         (unsigned long long)n, (unsigned long long)nnz, t_read, t_mv, m, reps);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|csbwrite|spmm <sys.xml> ...\n"); return 2; }
+  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|csbwrite|spmm|csbtimes <file> ...\n"); return 2; }
   std::string mode = argv[1];
+  if (mode == "csbtimes") return argc == 5 ? csbtimes(argv[2], atoi(argv[3]), atoi(argv[4])) : 2;
   Setup s;
   load(s, argv[2]);
   if (mode == "dumpint") {
