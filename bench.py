@@ -508,7 +508,12 @@ def cpu_reference_section(ca, op, n, k, reps=3):
         t_write = time.perf_counter() - t0
         print("[bench] cpu reference: CSB file written (%.1f GB) in %.1fs" % (os.path.getsize(path) / 1e9, t_write),
               file=sys.stderr, flush=True)
-        r = subprocess.run([exe, "csbtimes", path, str(k), str(reps)], capture_output=True, text=True, timeout=600)
+        try:
+            r = subprocess.run([exe, "csbtimes", path, str(k), str(reps)], capture_output=True, text=True,
+                               timeout=600)
+        except subprocess.TimeoutExpired:
+            print("[bench] cpu reference timed out", file=sys.stderr, flush=True)
+            return None
         if r.returncode != 0:
             print("[bench] cpu reference failed: %s" % r.stderr[-300:], file=sys.stderr, flush=True)
             return None
