@@ -510,7 +510,7 @@ def cpu_reference_section(ca, op, n, k, reps=3):
               file=sys.stderr, flush=True)
         try:
             r = subprocess.run([exe, "csbtimes", path, str(k), str(reps)], capture_output=True, text=True,
-                               timeout=600)
+                               timeout=150)
         except subprocess.TimeoutExpired:
             print("[bench] cpu reference timed out", file=sys.stderr, flush=True)
             return None
