@@ -202,7 +202,11 @@ def main():
         out["serve"] = serve_section(ca, synth)
     if not args.no_cpu:
         port = cpu_section(ca, op, n, k, args.cpu_rows)
-        ref = cpu_reference_section(ca, op, n, k)
+        try:  # a side measurement: it never stops the bench
+            ref = cpu_reference_section(ca, op, n, k)
+        except Exception as e:  # noqa: BLE001
+            print("[bench] cpu reference failed: %r" % (e,), file=sys.stderr, flush=True)
+            ref = None
         out["cpu_baseline"] = ref if ref is not None else port
         out["cpu_baseline_port"] = port
     print(json.dumps(out), flush=True)
@@ -322,8 +326,8 @@ def solve_sd6p6n_section(ca, synth, lims=(30, 90)):
     limits (30 = the reference's nume + 20); fastest FP64 run against fastest mixed run,
     and the per-eigenvalue error of the mixed runs against the FP64 runs."""
     import numpy as np
-    d = tempfile.mkdtemp(prefix="cosmo_sd6_")
-    s = ca.System.from_xml(synth.write_system(d, "sd", "sd6p6n", 6, 6, 0, "+", 1))
+    tdir = tempfile.mkdtemp(prefix="cosmo_sd6_")
+    s = ca.System.from_xml(synth.write_system(tdir, "sd", "sd6p6n", 6, 6, 0, "+", 1))
     op = ca.Operator.build(s)
     ca.davidson(op, 10, mblock=8, lim=lims[0], want_vectors=False, critr=1e-6, precision=ca.PREC_FP32)  # warm
     r, raw = {"n": int(s.dim), "block": 8, "pairs": 10}, {}
@@ -352,7 +356,51 @@ def solve_sd6p6n_section(ca, synth, lims=(30, 90)):
     r["mixed_speedup_same_lim"] = {str(L): round(r["fp64_lim%d" % L]["seconds"] / r["mixed_lim%d" % L]["seconds"], 3)
                                    for L in lims}
     op.close()
+    try:  # a side measurement: it never stops the bench
+        ref = cpu_reference_solve(tdir, "sd6p6n", 10)
+    except Exception as e:  # noqa: BLE001
+        print("[bench] cpu reference solve failed: %r" % (e,), file=sys.stderr, flush=True)
+        ref = None
+    if ref is not None:
+        e_ref = np.array(ref.pop("evals"))
+        ref["max_abs_dE_vs_gpu_fp64_MeV"] = float(np.max(np.abs(e_ref - raw[("fp64", best["fp64"])])))
+        # Davidson time alone on both sides (H built beforehand on both)
+        ref["gpu_mixed_speedup"] = round(ref["t_davidson_s"] / r["mixed_lim%d" % best["mixed"]]["seconds"], 1)
+        ref["gpu_fp64_speedup"] = round(ref["t_davidson_s"] / r["fp64_lim%d" % best["fp64"]]["seconds"], 1)
+        r["cpu_reference"] = ref
     return r
+
+
+def cpu_reference_solve(d, name, bloc):
+    """The reference's own path on this host for the system in d: XsysmbsOMP writes the
+    basis, then oracle/_ref/ref_harness builds the CSB H with PPMatrix2CSBBlk and runs
+    DavidsonDiagonalization (block = states = bloc, its defaults: critr 1e-8, lim bloc + 20)
+    with OpenMP.  None when those builds are absent."""
+    import json as _json
+    import subprocess
+    import numpy as np
+    import oracle
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    mbs, exe = os.path.join(ref, "XsysmbsOMP"), os.path.join(ref, "ref_harness")
+    if not (os.access(mbs, os.X_OK) and os.access(exe, os.X_OK)):
+        return None
+    try:
+        subprocess.run([mbs, name + ".xml"], cwd=d, capture_output=True, text=True, timeout=120, check=True)
+        t0 = time.perf_counter()
+        out = subprocess.run([exe, "csb", name + ".xml", "ref.EE", str(bloc), "0", str(bloc)], cwd=d,
+                             capture_output=True, text=True, timeout=150, check=True).stdout
+        wall = time.perf_counter() - t0
+    except (subprocess.SubprocessError, OSError) as e:
+        print("[bench] cpu reference solve skipped: %s" % e, file=sys.stderr, flush=True)
+        return None
+    info = _json.loads([ln for ln in out.splitlines() if ln.startswith("{")][-1])
+    E, dE, R, V = oracle.read_ee(os.path.join(d, "ref.EE"))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print("[bench] cpu reference solve: build %.2fs, Davidson %.2fs" % (info["t_build"], info["t_davidson"]),
+          file=sys.stderr, flush=True)
+    return {"cores": cores, "block": bloc, "pairs": bloc, "critr": 1e-8, "t_build_s": round(info["t_build"], 3),
+            "t_davidson_s": round(info["t_davidson"], 3), "wall_s": round(wall, 3),
+            "max_resid": float(np.max(R)) if len(R) else None, "evals": [float(x) for x in E[:bloc]]}
 
 
 def random_systems(ca, synth, count, seed, shells=("sd",), p_sd=1.0, max_sd=6, max_fp=4):
