@@ -195,6 +195,7 @@ struct Request {
   double tol;
   double t_submit, t_admit = 0, t_done = 0;
   int64_t bytes = 0;
+  int flags = 0;  // operator kind: stored, or the product form when H does not fit
 };
 
 struct cosmo_server_s {
@@ -249,7 +250,7 @@ static void server_worker(cosmo_server_s *S) {
     p.lim = res.nume + std::max(20, res.nume);
     p.maxiter = 4000;
     cosmo_dav_stats st;
-    int rc = res.n > 0 ? solve_one(r.sys, 0, p, res.nume, r.tol, res.evals, res.resid, &st) : COSMO_EINVAL;
+    int rc = res.n > 0 ? solve_one(r.sys, r.flags, p, res.nume, r.tol, res.evals, res.resid, &st) : COSMO_EINVAL;
     res.status = rc;
     res.t_submit = r.t_submit;
     res.t_admit = r.t_admit;
@@ -402,7 +403,18 @@ int cosmo_server_submit(cosmo_server S, cosmo_system sys, int nume, double tol, 
   int rc = stored_footprint(*sys, 0, &ob);
   if (rc) return rc;
   const int nm = (int)std::min<int64_t>(r.nume, std::max<int64_t>(sys->dim, 1));
-  r.bytes = ob + davidson_footprint(sys->dim, nm, nm, (int)std::min<int64_t>(batch_lim(nm), std::max<int64_t>(sys->dim, 1)));
+  const int64_t dav =
+      davidson_footprint(sys->dim, nm, nm, (int)std::min<int64_t>(batch_lim(nm), std::max<int64_t>(sys->dim, 1)));
+  r.bytes = ob + dav;
+  if (r.bytes > S->p.hbm_budget) {
+    // a Hamiltonian too large to store within the budget runs on the product form (no H
+    // in HBM) when the system has one
+    const int64_t fb = factored_footprint(*sys);
+    if (fb >= 0 && fb + dav <= S->p.hbm_budget) {
+      r.flags = COSMO_OP_FACTORED;
+      r.bytes = fb + dav;
+    }
+  }
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (r.bytes > S->p.hbm_budget) return fail(COSMO_ENOMEM, "request needs more HBM than the server budget");
@@ -452,6 +464,10 @@ int cosmo_server_destroy(cosmo_server S) {
 
 int cosmo_op_footprint(cosmo_system s, int flags, int64_t *bytes) {
   if (!s || !bytes) return fail(COSMO_EINVAL, "cosmo_op_footprint: null argument");
+  if (flags & COSMO_OP_FACTORED) {
+    *bytes = factored_footprint(*s);
+    return *bytes < 0 ? fail(COSMO_EUNSUP, "cosmo_op_footprint: no product form for this system") : COSMO_OK;
+  }
   if (flags & COSMO_OP_VIRTUAL) {
     *bytes = 8 * s->dim + (int64_t)s->orb.size() * s->orb.size() * s->orb.size() * s->orb.size();
     return COSMO_OK;

@@ -449,6 +449,43 @@ static int subsystem_rows(const System &sub, std::vector<int64_t> &rp, std::vect
 static inline int popc(uint32_t x) { return __builtin_popcount(x); }
 static inline uint32_t below32(int i) { return (1u << i) - 1u; }
 
+// Upper bound of the device bytes of a product-form operator of s together with the
+// class copies of X and Y its applies allocate at the widest Davidson pitch (16 FP64
+// vectors); -1 when s has no product form (one species, or terms that move a particle
+// between species, or sectors or dense class blocks beyond what build_factored takes).
+int64_t factored_footprint(const System &s) {
+  const int nn = s.nn, np = s.np, nblk = (int)s.nmask.size();
+  if (s.nv < 1 || s.zv < 1 || nn > 31 || np > 31) return -1;
+  for (size_t t = 0; t < s.esp_v.size(); ++t)
+    if ((s.esp_k[2 * t] < nn) != (s.esp_k[2 * t + 1] < nn)) return -1;
+  for (size_t t = 0; t < s.vspx_v.size(); ++t) {
+    const uint16_t *k = &s.vspx_k[4 * t];
+    if ((k[0] < nn) + (k[1] < nn) != (k[2] < nn) + (k[3] < nn)) return -1;
+  }
+  const int64_t n = s.dim;
+  const int nsec = (int)s.psec_start.size() - 1;
+  int64_t maxS = 0, dense = 0, hops = 0;
+  std::vector<int64_t> nb(nsec, 0);
+  for (int b = 0; b < nblk; ++b) {
+    maxS = std::max<int64_t>(maxS, s.noff[b + 1] - s.noff[b]);
+    if (s.nsector[b] >= 0 && s.nsector[b] < nsec) ++nb[s.nsector[b]];
+  }
+  if (maxS > 4096) return -1;
+  const int m2 = 2 * 31 + 1;
+  for (int sec = 0; sec < nsec; ++sec) {
+    const int64_t S = s.psec_start[sec + 1] - s.psec_start[sec];
+    dense += (S * S + nb[sec] * nb[sec]) * 12;  // Hp and Hn, FP64 + FP32
+    hops += S * ((int64_t)s.zv * (np - s.zv + 1) + 2 * m2 + 2) * 4 + 16;
+  }
+  if (dense > ((int64_t)4 << 30)) return -1;
+  int64_t wneg = 16;
+  while (wneg < (int64_t)np * np) wneg <<= 1;
+  const int64_t W = (int64_t)nn * nn * wneg * 2 * 12;
+  const int64_t steps = (int64_t)nblk * (1 + (int64_t)s.nv * (nn - s.nv + 1)) * 12 + 64 * (int64_t)nblk;
+  const int64_t tables = (int64_t)(nn + np) * (nn + np) * (nn + np) * (nn + np) * 8 + ((int64_t)1 << 20);
+  return 4 * n * 16 * 8 + 8 * n + dense + hops + W + steps + tables;
+}
+
 int build_factored(Op &op, const System &s) {
   const int nn = s.nn, np = s.np;
   const int nblk = (int)s.nmask.size();

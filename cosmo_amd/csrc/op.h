@@ -39,6 +39,7 @@ int upper_to_full_dev(Op &op, int64_t n, const int64_t *rowptr, const int64_t *c
 // host -> device copy through pinned double buffers
 int h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st);
 int build_factored(Op &op, const System &s);
+int64_t factored_footprint(const System &s);  // -1: no product form
 void free_factored(Op &op);
 size_t factored_bytes(const Op &op);
 int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);

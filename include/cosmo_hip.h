@@ -269,7 +269,9 @@ typedef struct {
 
 int cosmo_server_create(const cosmo_server_params *p, cosmo_server *out);
 /* Queue a request: the system (ownership stays with the caller until the request
- * completes), pairs wanted and residual tolerance.  Returns the request id in *id. */
+ * completes), pairs wanted and residual tolerance.  Returns the request id in *id.  A
+ * request whose stored H and Davidson blocks exceed hbm_budget runs on the product-form
+ * operator (COSMO_OP_FACTORED, no H in HBM) when that fits; otherwise COSMO_ENOMEM. */
 int cosmo_server_submit(cosmo_server s, cosmo_system sys, int nume, double tol, int64_t *id);
 /* Run admission + one scheduling round; returns the number of results written (<= cap).*/
 int cosmo_server_poll(cosmo_server s, cosmo_result *results, int cap);

@@ -153,3 +153,31 @@ def test_single_launch_tiny_and_full_space(gpu):
         assert np.max(r[:k]) <= 1e-8
         assert np.all(e[k:] == 0.0)
     b.close()
+
+
+def test_server_runs_unstorable_h_on_the_product_form(gpu, tmp_path):
+    """A request whose stored H does not fit the server's HBM budget, but whose product form
+    (no H in HBM) does, is served on the product form: its eigenvalues equal a stored solve
+    alone to 1e-9 and its residuals meet the tolerance.  A request that fits neither way is
+    refused at submission, and a one-species system has no product form."""
+    xml = synth.write_system(str(tmp_path), "fp", "u", 3, 3, 0, "+", 4)
+    s = gpu.System.from_xml(xml)
+    k = 4
+    stored, fact = gpu.footprint(s), gpu.footprint(s, gpu.OP_FACTORED)
+    assert 0 < 3 * fact < stored  # the budget below holds the product form and its blocks only
+    lim = batch_lim(k, s.dim)
+    ea = gpu.davidson(gpu.Operator.build(s), k, want_vectors=False, critr=1e-8, lim=lim, maxiter=4000)[0]
+    srv = gpu.Server(hbm_budget=stored, max_active=2, precision=gpu.PREC_FP64)
+    srv.submit(s, k, 1e-8)
+    res = srv.drain()
+    srv.close()
+    assert len(res) == 1 and res[0]["status"] == 0
+    assert np.max(np.abs(res[0]["evals"] - ea)) < 1e-9
+    assert np.max(res[0]["resid"]) <= 1e-8
+    small = gpu.Server(hbm_budget=1 << 20, max_active=1, precision=gpu.PREC_FP64)
+    with pytest.raises(gpu.CosmoError):
+        small.submit(s, k, 1e-8)
+    small.close()
+    one = gpu.System.from_xml(synth.write_system(str(tmp_path), "sd", "one", 0, 4, 0, "+", 2))
+    with pytest.raises(gpu.CosmoError):
+        gpu.footprint(one, gpu.OP_FACTORED)
