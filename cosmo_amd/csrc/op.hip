@@ -517,46 +517,79 @@ int cosmo_op_write_csb(cosmo_op op, const char *path, int block) {
   fwrite(&b16, 2, 1, f);
   fwrite(&nb16, 2, 1, f);
   fwrite(&nb16, 2, 1, f);
-  std::vector<int64_t> cnt, rp, cols, start(nb + 1);
-  std::vector<double> vals, bv;
+  hipSetDevice(op->device);
+  hipStream_t st = lib_stream();
+  const CsrDev &M = op->csr;
+  auto bad = [&](const std::string &m) {
+    fclose(f);
+    return fail(COSMO_EIO, m);
+  };
+  // per block row: one copy of its stored rows, their upper parts (the tail of each
+  // ascending row from the diagonal on) grouped by block column, and every block of the
+  // block row laid out in one buffer and written at once
+  std::vector<int64_t> rp, start(nb + 1), at(nb);
+  std::vector<uint32_t> hc;
+  std::vector<double> hv;
+  std::vector<int64_t> first;  // first upper entry of each row of the block row
   std::vector<uint16_t> bc, brow;
+  std::vector<double> bv;
+  std::vector<char> buf;
   for (int64_t I = 0; I < nb; ++I) {
     const int64_t r0 = I * br, r1 = std::min(n, r0 + br);
-    cnt.assign(r1 - r0, 0);
-    int rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), nullptr, nullptr, nullptr);
-    rp.assign(r1 - r0 + 1, 0);
-    for (int64_t i = 0; i < r1 - r0; ++i) rp[i + 1] = rp[i] + cnt[i];
-    cols.resize(rp.back());
-    vals.resize(rp.back());
-    if (!rc) rc = cosmo_op_upper_rows(op, r0, r1, cnt.data(), rp.data(), cols.data(), vals.data());
-    if (rc) {
-      fclose(f);
-      return rc;
-    }
-    // entries of the block row grouped by block column, each group in (row, col) order
+    rp.resize(r1 - r0 + 1);
+    if (hipMemcpyAsync(rp.data(), M.rowptr + r0, sizeof(int64_t) * rp.size(), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return bad("reading the stored rows failed");
+    const int64_t e0 = rp[0], e1 = rp.back();
+    hc.resize(e1 - e0);
+    hv.resize(e1 - e0);
+    if (e1 > e0 &&
+        (hipMemcpyAsync(hc.data(), M.cols + e0, sizeof(uint32_t) * hc.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(hv.data(), M.v64 + e0, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+      return bad("reading the stored rows failed");
+    first.resize(r1 - r0);
     std::fill(start.begin(), start.end(), 0);
-    for (int64_t e = 0; e < rp.back(); ++e) start[cols[e] / br + 1]++;
+    int64_t nup = 0;
+    for (int64_t i = 0; i < r1 - r0; ++i) {
+      int64_t a = rp[i] - e0;
+      const int64_t b = rp[i + 1] - e0;
+      while (a < b && (int64_t)hc[a] < r0 + i) ++a;
+      first[i] = a;
+      for (int64_t e = a; e < b; ++e) start[hc[e] / br + 1]++;
+      nup += b - a;
+    }
     for (int64_t J = 0; J < nb; ++J) start[J + 1] += start[J];
-    bc.resize(rp.back());
-    brow.resize(rp.back());
-    bv.resize(rp.back());
-    std::vector<int64_t> at(start.begin(), start.end() - 1);
+    bc.resize(nup);
+    brow.resize(nup);
+    bv.resize(nup);
+    std::copy(start.begin(), start.end() - 1, at.begin());
     for (int64_t i = 0; i < r1 - r0; ++i)
-      for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
-        const int64_t J = cols[e] / br, q = at[J]++;
-        bc[q] = (uint16_t)(cols[e] - J * br);
+      for (int64_t e = first[i]; e < rp[i + 1] - e0; ++e) {
+        const int64_t J = hc[e] / br, q = at[J]++;
+        bc[q] = (uint16_t)(hc[e] - J * br);
         brow[q] = (uint16_t)i;
-        bv[q] = vals[e];
+        bv[q] = hv[e];
       }
+    buf.resize((size_t)(nb * (sizeof(bool) + 4) + nup * 12));
+    char *w = buf.data();
     for (int64_t J = 0; J < nb; ++J) {
       const bool diag = J == I;
       const uint32_t nnz = (uint32_t)(start[J + 1] - start[J]);
-      fwrite(&diag, sizeof(bool), 1, f);
-      fwrite(&nnz, 4, 1, f);
-      fwrite(bc.data() + start[J], 2, nnz, f);
-      fwrite(brow.data() + start[J], 2, nnz, f);
-      fwrite(bv.data() + start[J], 8, nnz, f);
+      std::memcpy(w, &diag, sizeof(bool));
+      w += sizeof(bool);
+      std::memcpy(w, &nnz, 4);
+      w += 4;
+      std::memcpy(w, bc.data() + start[J], 2 * (size_t)nnz);
+      w += 2 * (size_t)nnz;
+      std::memcpy(w, brow.data() + start[J], 2 * (size_t)nnz);
+      w += 2 * (size_t)nnz;
+      std::memcpy(w, bv.data() + start[J], 8 * (size_t)nnz);
+      w += 8 * (size_t)nnz;
     }
+    const size_t len = (size_t)(w - buf.data());
+    if (fwrite(buf.data(), 1, len, f) != len) return bad(std::string("write failed: ") + path);
   }
   if (fclose(f)) return fail(COSMO_EIO, std::string("write failed: ") + path);
   return COSMO_OK;
