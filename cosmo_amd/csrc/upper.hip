@@ -195,15 +195,21 @@ int upper_to_full_dev(Op &op, int64_t n, const int64_t *rowptr, const int64_t *c
     return rc;
   }
   const unsigned grid = (unsigned)std::max<int64_t>(1, (n + 255) / 256);
-  hipMemsetAsync(lowcnt, 0, sizeof(int32_t) * std::max<int64_t>(n, 1), st);
-  hipMemsetAsync(err, 0, sizeof(int), st);
+  // every queued call's status is folded into e; the synchronisation below reports it
+  hipError_t e = hipMemsetAsync(lowcnt, 0, sizeof(int32_t) * std::max<int64_t>(n, 1), st);
+  auto also = [&](hipError_t x) {
+    if (e == hipSuccess) e = x;
+  };
+  also(hipMemsetAsync(err, 0, sizeof(int), st));
   if (n) hipLaunchKernelGGL(k_sym_count, dim3(grid), dim3(256), 0, st, n, urp, uc, uv, keep, upcnt, lowcnt, err);
+  also(hipGetLastError());
   std::vector<int32_t> hu(n), hl(n);
   int herr = 0;
-  hipMemcpyAsync(hu.data(), upcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
-  hipMemcpyAsync(hl.data(), lowcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
-  hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (hipStreamSynchronize(st) != hipSuccess) {
+  also(hipMemcpyAsync(hu.data(), upcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  also(hipMemcpyAsync(hl.data(), lowcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  also(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+  also(hipStreamSynchronize(st));
+  if (e != hipSuccess) {
     cleanup();
     return fail(COSMO_EHIP, "upper rows: count pass");
   }
@@ -235,8 +241,8 @@ int upper_to_full_dev(Op &op, int64_t n, const int64_t *rowptr, const int64_t *c
     cleanup();
     return fail(COSMO_ENOMEM, "upper rows: operator arrays");
   }
-  hipMemcpyAsync(M.rowptr, frp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st);
-  hipMemsetAsync(upcnt, 0, sizeof(int32_t) * std::max<int64_t>(n, 1), st);  // reused as the lower fill counters
+  also(hipMemcpyAsync(M.rowptr, frp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, st));
+  also(hipMemsetAsync(upcnt, 0, sizeof(int32_t) * std::max<int64_t>(n, 1), st));  // reused as the lower fill counters
   if (n) {
     hipLaunchKernelGGL(k_sym_scatter, dim3(grid), dim3(256), 0, st, n, urp, uc, uv, keep, M.rowptr, lowcnt, upcnt,
                        M.cols, M.v64, M.diag);
@@ -250,12 +256,12 @@ int upper_to_full_dev(Op &op, int64_t n, const int64_t *rowptr, const int64_t *c
     }
     if (nnz) hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, st, nnz, M.v64, M.v32);
   }
-  const hipError_t le = hipGetLastError();
+  also(hipGetLastError());
   op.hdiag.assign(n, 0.0);
-  if (n) hipMemcpyAsync(op.hdiag.data(), M.diag, sizeof(double) * n, hipMemcpyDeviceToHost, st);
-  const hipError_t se = hipStreamSynchronize(st);
+  if (n) also(hipMemcpyAsync(op.hdiag.data(), M.diag, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  also(hipStreamSynchronize(st));
   cleanup();
-  if (le != hipSuccess || se != hipSuccess) return fail(COSMO_EHIP, "upper rows: assembly");
+  if (e != hipSuccess) return fail(COSMO_EHIP, std::string("upper rows: assembly: ") + hipGetErrorString(e));
   return COSMO_OK;
 }
 
