@@ -1,4 +1,4 @@
-// Drives cosmo::GpuMatrix (include/cosmo_matrixbase.hpp) through the members that
+// Exercises cosmo::GpuMatrix (include/cosmo_matrixbase.hpp) through the members that
 // MyMatrixDiagonal<SomeMatrix> (include/kdav/MandD.cxx:7-42) uses: copy construction,
 // size(), ReadDiagonal(double*), TimesMatrix(double**, double**, int) and
 // TimesMatrix(double*, double*, int) -- the in-process boundary a CoSMo build would bind.

@@ -26,7 +26,7 @@ def _compile(tmp_path):
 
 
 def test_adapter_compiles_against_the_contract(tmp_path):
-    """The header-only adapter and the driver that exercises it build with the host
+    """The header-only adapter and the test program that exercises it build with the host
     compiler alone (no HIP toolchain needed by a CoSMo build)."""
     if not os.path.exists(os.path.join(ROOT, "cosmo_amd", "libcosmo_hip.so")):
         pytest.skip("library not built")
