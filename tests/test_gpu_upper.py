@@ -90,3 +90,48 @@ def test_upper_rows_must_ascend(gpu):
         rp, c = np.array(rp, np.int64), np.array(c, np.int64)
         with pytest.raises(gpu.CosmoError):
             gpu.Operator.from_upper(3, rp, c, np.ones(len(c)))
+
+
+def test_large_upload_through_pinned_staging(gpu):
+    """Upper rows larger than one 32 MB staging buffer (5 M entries: 40 MB of columns and
+    40 MB of values) go through the pinned double buffers; the product equals the same
+    symmetric matrix's product in scipy to 1e-12."""
+    import scipy.sparse as sp
+    n, per = 200000, 50
+    rng = np.random.default_rng(8)
+    c = np.sort(rng.integers(0, n, size=(n, per)), axis=1)
+    rows = np.repeat(np.arange(n), per)
+    c = c.ravel()
+    keep = c > rows  # strictly upper, then the diagonal added per row
+    r_u, c_u = rows[keep], c[keep]
+    A = sp.coo_matrix((rng.standard_normal(len(r_u)), (r_u, c_u)), shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    A = A + sp.diags(rng.standard_normal(n))
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    assert A.nnz > 4_200_000
+    op = gpu.Operator.from_upper(n, A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data,
+                                 gpu.OP_KEEP_SMALL)
+    H = A + sp.triu(A, k=1).T
+    X = rng.standard_normal((n, 3))
+    ref = H @ X
+    y = op.apply(X, gpu.PREC_FP64)
+    assert np.max(np.abs(y - ref)) < 1e-12 * np.abs(ref).max()
+
+
+def test_read_hh_rejects_bad_files(gpu, golden, tmp_path):
+    """A .HH file cut short, with a row count beyond its size, or with bytes after the last
+    row is an error."""
+    g = golden("sd2p2n_s11")
+    op = gpu.Operator.build(gpu.System.from_xml(g.xml), gpu.OP_KEEP_SMALL)
+    p = str(tmp_path / "h.HH")
+    op.write_hh(p)
+    good = open(p, "rb").read()
+    bad = {"cut": good[:-5], "trail": good + b"\0" * 16,
+           "rows": np.array([10 ** 9], "<u8").tobytes() + good[8:]}
+    for k, data in bad.items():
+        q = str(tmp_path / (k + ".HH"))
+        open(q, "wb").write(data)
+        with pytest.raises(gpu.CosmoError):
+            gpu.Operator.read_hh(q)
+    assert gpu.Operator.read_hh(p).n == op.n
