@@ -920,7 +920,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   std::vector<double> rnorm(nume, 0.0);
   bool first = true, omin_to = false, fp64_phase = !mixed;
   int nloops = 0, ierr = 0, restarts = 0;
-  // mixed precision: collapse to the Ritz vectors, recompute H.B in FP64, restart
+  // mixed precision: widen the basis to FP64, recompute H.B in FP64, continue in FP64
   auto switch_fp64 = [&]() -> int {
     // keep the whole FP32-grown basis; it is re-orthonormalised in FP64 (vectors added
     // with a single round may have drifted, which matters once the basis nears the whole

@@ -1,7 +1,7 @@
 """Writes the golden fixtures of tests/golden from the reference's own programs.
 
-Run in a container that has the CoSMo source tree at /root/reference (never on the GPU
-box): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
+Run only where the CoSMo source tree is present at /root/reference (the committed
+fixtures are what the tests read elsewhere): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
 into oracle/_ref from those sources, then this script runs them on seeded synthetic
 systems and stores inputs and outputs as small data files (npz / json / text).
 ``python make_golden.py csb`` only adds the CSB file keys to the existing JSON files.
