@@ -8,7 +8,9 @@
 * the stored product is symmetric (y'Hx = x'Hy) and linear, the FP32 product is within
   FP32 rounding of the FP64 one, and the product form (no H in memory) gives the same
   FP64 products;
-* the diagonal read back equals the unit-vector products' diagonal entries.
+* the diagonal read back equals the unit-vector products' diagonal entries;
+* the 10-pair mixed-precision solve is within 1 keV of the all-FP64 one, with residuals
+  at most 1e-6 as the solver reports them.
 """
 import numpy as np
 import pytest
@@ -102,3 +104,26 @@ def test_products_symmetric_linear_and_consistent(gpu, full):
     finally:
         fop.close()
     assert np.max(np.abs(F - Y)) <= 1e-12 * scale
+
+
+def test_ten_pair_solve_mixed_vs_fp64(gpu, full):
+    """The 10-pair solve (block 16, residual 1e-6) on the product form: the mixed-precision
+    eigenvalues within 1 keV of the all-FP64 ones, and every returned pair's residual
+    recomputed here with the stored FP64 product, independently of the solver."""
+    s, op, _, _ = full
+    fop = gpu.Operator.build(s, gpu.OP_FACTORED)
+    try:
+        kw = dict(mblock=16, lim=60, critr=1e-6)
+        e64, _, r64, _, st64 = gpu.davidson(fop, 10, want_vectors=False, precision=gpu.PREC_FP64, **kw)
+        e32, _, r32, V, st32 = gpu.davidson(fop, 10, want_vectors=True, precision=gpu.PREC_FP32, **kw)
+    finally:
+        fop.close()
+    assert st64["converged"] and st32["converged"]
+    assert np.max(r64) <= 1e-6 and np.max(r32) <= 1e-6
+    assert np.all(np.diff(e64) >= 0) and np.all(np.diff(e32) >= 0)
+    assert np.max(np.abs(e32 - e64)) <= 1e-3  # MeV: 1 keV
+    assert st32["matvecs_fp64"] < st32["matvecs"]  # the mixed solve ran an FP32 phase
+    HV = op.apply(V, gpu.PREC_FP64)
+    assert np.max(np.abs(V.T @ V - np.eye(10))) <= 1e-8
+    res = np.linalg.norm(HV - V * e32[None, :], axis=0)
+    assert np.max(res) <= 2e-6, res
