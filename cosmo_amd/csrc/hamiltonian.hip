@@ -440,11 +440,15 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t *__restri
 // loads one (col, value) pair (coalesced, prefetched one chunk ahead), then LPE steps
 // broadcast the entries to lane groups of LPE lanes, each lane gathering one 16-byte
 // piece of its entry's X row.  Partial sums are reduced across groups at the row end.
-// Workgroups are remapped so that each XCD (dispatched round-robin) sweeps a contiguous
-// range of rows: neighbouring rows touch neighbouring X rows, which then share an L2.  A
-// workgroup holds 16 consecutive rows (1024 threads), so they also share one CU's L1:
-// measured at fp 4p4n, k=16, 4 / 8 / 16 rows per workgroup take 7.60 / 7.14 / 6.89 ms
-// (FP32) and 11.31 / 11.57 / 11.15 ms (FP64), same bits.
+// Workgroups are remapped so that each XCD (dispatched round-robin) takes chunks of 64
+// consecutive workgroups: neighbouring rows touch neighbouring X rows, which then share
+// that XCD's L2, and the eight XCDs work on nearby chunks at once, so the X rows they
+// gather overlap in the shared Infinity Cache.  A workgroup holds 16 consecutive rows
+// (1024 threads), so they also share one CU's L1.  Measured at fp 4p4n, k=16, FP32 /
+// FP64, same bits in every case: 4 / 8 / 16 rows per workgroup 7.60 / 7.14 / 6.89 ms and
+// 11.31 / 11.57 / 11.15 ms with one contiguous eighth of the rows per XCD; 16 rows with
+// chunks of 16 / 64 / 512 / 4096 workgroups 6.79 / 6.74 / 6.76 / 6.85 ms (FP32); 2 or 4
+// rows per wave in sequence (32 or 64 rows per workgroup) 7.55 / 8.22 ms.
 template <typename T>
 struct Vec16;
 template <>
@@ -461,8 +465,13 @@ struct Vec16<double> {
 __device__ __forceinline__ int64_t xcd_row_block(int remap) {
   const int64_t G = gridDim.x, i = blockIdx.x;
   if (!remap) return i;
-  const int64_t q = G / 8, r = G % 8, x = i % 8;
-  return x * q + (x < r ? x : r) + i / 8;
+  // XCD x = i % 8 takes chunks x, x + 8, x + 16, ... of C workgroups; the last
+  // G mod 8C workgroups, dispatched last, keep their own index (a bijection on [0, G))
+  constexpr int64_t C = 64;
+  const int64_t G8 = (G / (8 * C)) * (8 * C);
+  if (i >= G8) return i;
+  const int64_t x = i % 8, j = i / 8;
+  return ((j / C) * 8 + x) * C + j % C;
 }
 
 template <typename T, int LPE>
@@ -821,7 +830,7 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
 
 // ------------------------------------------------------------------------------------
 // products
-int g_spmm_remap = 1;  // XCD-contiguous row ranges (COSMO_SPMM_REMAP=0 turns it off)
+int g_spmm_remap = 1;  // XCD-chunked row ranges (COSMO_SPMM_REMAP=0 turns it off)
 
 // vectorised kernel when X and Y share a power-of-two leading dimension of >= 16 bytes
 template <typename T>
