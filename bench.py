@@ -403,34 +403,12 @@ def cpu_reference_solve(d, name, bloc):
             "max_resid": float(np.max(R)) if len(R) else None, "evals": [float(x) for x in E[:bloc]]}
 
 
-def random_systems(ca, synth, count, seed, shells=("sd",), p_sd=1.0, max_sd=6, max_fp=4):
-    """Seeded random requests: shell, Nv, Zv, 2Jz from the allowed range, own interaction."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    d = tempfile.mkdtemp(prefix="cosmo_req_")
-    out = []
-    while len(out) < count:
-        shell = "sd" if rng.random() < p_sd else "fp"
-        cap = max_sd if shell == "sd" else max_fp
-        nv, zv = int(rng.integers(1, cap + 1)), int(rng.integers(1, cap + 1))
-        A = nv + zv
-        jmax = 2 * A  # generous; empty M-spaces are redrawn
-        twojz = int(rng.choice([j for j in range(-jmax, jmax + 1) if (j - A) % 2 == 0]))
-        par = "+" if shell == "sd" else ("-" if A % 2 else "+")
-        xml = synth.write_system(d, shell, "r%d" % len(out), nv, zv, twojz, par, int(rng.integers(1 << 30)))
-        s = ca.System.from_xml(xml)
-        if s.dim < 1:
-            continue
-        out.append((shell, nv, zv, twojz, s))
-    return out, rng
-
-
 def batch_section(ca, synth):
     """256 independent sd-shell solves of uneven size in one batch, mixed precision."""
     import numpy as np
     t0 = time.perf_counter()
-    reqs, _ = random_systems(ca, synth, 256, 17)
-    systems = [r[4] for r in reqs]
+    reqs, _ = synth.random_requests(ca, 256, 17)
+    systems = [r[5] for r in reqs]
     nume = [min(5, s.dim) for s in systems]
     t_setup = time.perf_counter() - t0
     print("[bench] batch: %d systems set up in %.2fs, building operators" % (len(systems), t_setup),
@@ -457,8 +435,8 @@ def batch_section(ca, synth):
 def serve_section(ca, synth, count=48, rate=12.0):
     """Poisson arrivals through the server; latency, throughput and FP64 agreement."""
     import numpy as np
-    reqs, rng = random_systems(ca, synth, count, 23, p_sd=0.7, max_sd=6, max_fp=4)
-    ks = [int(min(rng.integers(1, 11), r[4].dim)) for r in reqs]
+    reqs, rng = synth.random_requests(ca, count, 23, p_sd=0.7, max_sd=6, max_fp=4)
+    ks = [int(min(rng.integers(1, 11), r[5].dim)) for r in reqs]
     tols = [float(10 ** rng.uniform(-6, -4)) for _ in reqs]
     gaps = rng.exponential(1.0 / rate, size=len(reqs))
     print("[bench] serve: %d requests at %.1f/s" % (len(reqs), rate), file=sys.stderr, flush=True)
@@ -467,7 +445,7 @@ def serve_section(ca, synth, count=48, rate=12.0):
     t0 = time.perf_counter()
     for (r, k, tol, gap) in zip(reqs, ks, tols, gaps):
         time.sleep(float(gap))
-        ids.append(srv.submit(r[4], k, tol))
+        ids.append(srv.submit(r[5], k, tol))
     res = srv.drain()
     wall = time.perf_counter() - t0
     srv.close()
@@ -478,7 +456,7 @@ def serve_section(ca, synth, count=48, rate=12.0):
     # accuracy of each result against an all-FP64 solve of the same request alone
     dE = []
     for i, (r, k, tol) in zip(ids, zip(reqs, ks, tols)):
-        s = r[4]
+        s = r[5]
         if s.dim > 100000:
             continue
         o = ca.Operator.build(s)
@@ -492,7 +470,7 @@ def serve_section(ca, synth, count=48, rate=12.0):
           file=sys.stderr, flush=True)
     shells = [r[0] for r in reqs]
     return {"requests": len(ids), "rate_per_s": rate, "sd_fraction": round(shells.count("sd") / len(shells), 3),
-            "dims_max": int(max(r[4].dim for r in reqs)), "wall_s": round(wall, 3),
+            "dims_max": int(max(r[5].dim for r in reqs)), "wall_s": round(wall, 3),
             "solves_per_s": round(len(ids) / wall, 3), "eigenpairs_per_s": round(sum(ks) / wall, 3),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 4),

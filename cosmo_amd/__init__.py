@@ -29,6 +29,15 @@ class CosmoError(RuntimeError):
         self.code = code
 
 
+class NotConverged(CosmoError):
+    """The eigensolver stopped with pairs above the residual target (COSMO_ECONV).
+    ``results`` holds what the call would have returned."""
+
+    def __init__(self, msg, results):
+        super().__init__(ECONV, msg)
+        self.results = results
+
+
 class DavParams(ctypes.Structure):
     _fields_ = [("nume", ctypes.c_int), ("mblock", ctypes.c_int), ("lim", ctypes.c_int), ("maxiter", ctypes.c_int),
                 ("crite", ctypes.c_double), ("critc", ctypes.c_double), ("critr", ctypes.c_double),
@@ -351,8 +360,12 @@ def dav_params(nume, **kw):
     return p
 
 
-def davidson(op: Operator, nume, init=None, want_vectors=True, **kw):
-    """Lowest nume eigenpairs: returns (evals, dE, resid, evecs (n x nume) or None, stats dict)."""
+def davidson(op: Operator, nume, init=None, want_vectors=True, check_converged=True, **kw):
+    """Lowest nume eigenpairs: returns (evals, dE, resid, evecs (n x nume) or None, stats dict).
+
+    A solve that stops with a pair above critr (maxiter, orthogonalisation failure, ...)
+    raises NotConverged carrying the results, unless check_converged is False; the stats
+    dict's ``converged`` says which case it was."""
     p = dav_params(nume, **kw)
     n = op.n
     ev = np.zeros(nume)
@@ -367,7 +380,10 @@ def davidson(op: Operator, nume, init=None, want_vectors=True, **kw):
     rc = lib().cosmo_davidson(op.h, ctypes.byref(p), _p(ini), _p(ev), _p(de), _p(rs), _p(vec), ctypes.byref(st))
     if rc != ECONV:
         check(rc)
-    return ev, de, rs, (vec.T if vec is not None else None), dict(st.as_dict(), converged=(rc == OK))
+    out = (ev, de, rs, (vec.T if vec is not None else None), dict(st.as_dict(), converged=(rc == OK)))
+    if rc == ECONV and check_converged:
+        raise NotConverged(lib().cosmo_last_error().decode(), out)
+    return out
 
 
 def write_ee(path, evals, dE, resid, evecs):
@@ -397,9 +413,10 @@ class Batch:
         """Dimension bound of the one-launch path for small solves (0: off)."""
         check(lib().cosmo_batch_set_single_launch(self.h, ctypes.c_int64(int(nmax))))
 
-    def solve(self, nume, **kw):
+    def solve(self, nume, check_converged=True, **kw):
         """nume: int or per-system list.  Returns (list of eigenvalue arrays, list of
-        residual arrays, stats dict)."""
+        residual arrays, stats dict); raises NotConverged (results attached) when a solve
+        stopped above its residual target, unless check_converged is False."""
         nl = [nume] * len(self.systems) if np.isscalar(nume) else list(nume)
         p = dav_params(max(nl), **kw)
         cnt = np.array(nl, dtype=np.int32)
@@ -411,8 +428,11 @@ class Batch:
         if rc != ECONV:
             check(rc)
         off = np.concatenate([[0], np.cumsum(cnt)])
-        return ([ev[off[i]:off[i + 1]] for i in range(len(nl))], [rs[off[i]:off[i + 1]] for i in range(len(nl))],
-                dict(st.as_dict(), converged=(rc == OK)))
+        out = ([ev[off[i]:off[i + 1]] for i in range(len(nl))], [rs[off[i]:off[i + 1]] for i in range(len(nl))],
+               dict(st.as_dict(), converged=(rc == OK)))
+        if rc == ECONV and check_converged:
+            raise NotConverged(lib().cosmo_last_error().decode(), out)
+        return out
 
     def close(self):
         if self.h:

@@ -366,7 +366,7 @@ int cosmo_batch_solve(cosmo_batch b, const cosmo_dav_params *p, const int *nume_
   for (int i = 0; i < ns; ++i)
     if (rcs[i] && rcs[i] != COSMO_ECONV) return rcs[i];
   for (int i = 0; i < ns; ++i)
-    if (rcs[i]) return fail(COSMO_ECONV, "a batched solve reached maxiter");
+    if (rcs[i]) return fail(COSMO_ECONV, "batched solve " + std::to_string(i) + " stopped above its residual target");
   return COSMO_OK;
 }
 

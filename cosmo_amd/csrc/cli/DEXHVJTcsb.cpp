@@ -85,6 +85,8 @@ int main(int argc, char **argv) {
     int rc = cosmo_davidson(op, &p, init.empty() ? nullptr : init.data(), ev.data(), de.data(), rs.data(),
                             vec.data(), &st);
     if (rc && rc != COSMO_ECONV) return die("davidson");
+    // the reference writes its .EE whatever the residuals; so does this tool, with a warning
+    if (rc == COSMO_ECONV) fprintf(stderr, "D: warning: %s\n", cosmo_last_error());
     count = nstate ? nstate : bloc;
     for (int i = 0; i < bloc; ++i) printf("%.6g %.6g %.6g\n", ev[i], de[i], rs[i]);
   } else {

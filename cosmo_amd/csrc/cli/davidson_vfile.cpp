@@ -50,6 +50,8 @@ int main(int argc, char **argv) {
   cosmo_dav_stats st;
   int rc = cosmo_davidson(op, &p, init.empty() ? nullptr : init.data(), ev.data(), de.data(), rs.data(), vec.data(), &st);
   if (rc && rc != COSMO_ECONV) return die("davidson");
+  // the reference writes its .EE whatever the residuals; so does this tool, with a warning
+  if (rc == COSMO_ECONV) fprintf(stderr, "D: warning: %s\n", cosmo_last_error());
   const std::string outf = sysfile + ".EE";
   const int save = nstate ? nstate : bloc;
   fprintf(stderr, " writing output to %s\n", outf.c_str());

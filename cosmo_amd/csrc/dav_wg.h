@@ -747,6 +747,13 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
     s.info[2] = sh.matvecs64;
     s.info[3] = sh.restarts;
     s.info[4] = sh.ierr;
-    s.info[5] = (sh.nloops >= s.maxiter && !sh.omin_to) ? 0 : 1;
+    // converged: stopped before maxiter and, under the strict test, every returned
+    // residual below critr (an orthogonalisation failure or the reopen cap counts as not
+    // converged)
+    int conv = (sh.nloops >= s.maxiter && !sh.omin_to) ? 0 : 1;
+    if (prm.strict)
+      for (int i = 0; i < nume; ++i)
+        if (!(sh.rnorm[i] <= s.critr)) conv = 0;
+    s.info[5] = conv;
   }
 }

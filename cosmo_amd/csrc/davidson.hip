@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -294,6 +295,34 @@ __global__ void k_rows_to_cols(int64_t n, int m, const T *__restrict__ Y, int ld
   B[(int64_t)j * ldb + i] = (V)Y[i * ldy + j];
 }
 
+// Pinned, device-mapped host staging (Gram read-back, then two upload slots), one per
+// running solve, from a library-wide pool: a solve takes one and gives it back when it
+// ends.  Buffers are kept until the process ends (freeing pinned memory synchronises the
+// device, which would stall concurrent solves), so the pool holds as many as solves ever
+// ran at once, whichever threads ran them.  Coherent (fine-grained): the Gram sums the
+// device writes are visible to the host as soon as the stream has drained.
+static std::mutex g_stage_mu;
+static std::vector<double *> g_stage_free;
+static double *stage_acquire() {
+  {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!g_stage_free.empty()) {
+      double *p = g_stage_free.back();
+      g_stage_free.pop_back();
+      return p;
+    }
+  }
+  double *p = nullptr;
+  if (hipHostMalloc((void **)&p, sizeof(double) * (GRAM_SMALL + 2 * GRAM_UPLOAD),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return nullptr;
+  return p;
+}
+static void stage_release(double *p) {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  g_stage_free.push_back(p);
+}
+
 // ------------------------------------------------------------------------------------
 struct Dav {
   Op *op = nullptr;
@@ -317,11 +346,17 @@ struct Dav {
   int G = 0;
   double ms_apply = 0, ms_ortho = 0, ms_rr = 0;
   int matvecs = 0, matvecs64 = 0;
+  int verbose = 0;
   ~Dav() {
     for (void *p : {(void *)B, (void *)AB, (void *)R, (void *)T, (void *)Xd, (void *)Yd, (void *)Xf, (void *)Yf,
                     (void *)B32, (void *)AB32, (void *)R32, (void *)T32, (void *)partial, (void *)segs,
                     (void *)small})
       dfree(p);
+    if (hbuf) {
+      // work queued before an early error return may still write the buffer
+      if (st) (void)hipStreamSynchronize(st);
+      stage_release(hbuf);
+    }
   }
   double *col(double *base, int j) { return base + (int64_t)j * n; }
   void *cv(void *base, int j) const { return (char *)base + (int64_t)j * n * es; }
@@ -364,15 +399,8 @@ static int dav_alloc(Dav &D, bool mixed) {
   HIPCHK(dmalloc(&D.partial, sizeof(double) * (size_t)D.G * GRAM_MAXP * GRAM_MAXQ));
   HIPCHK(dmalloc(&D.segs, sizeof(double) * (size_t)GRAM_SEG * GRAM_MAXP * GRAM_MAXQ));
   HIPCHK(dmalloc(&D.small, sizeof(double) * GRAM_SMALL));
-  // the pinned staging buffer belongs to the host thread and is never freed: freeing
-  // pinned memory synchronises the device, which would stall concurrent solves.  It is
-  // coherent (fine-grained), so the Gram sums the device writes into it are visible to
-  // the host as soon as the stream has drained.
-  static thread_local double *hstage = nullptr;
-  if (!hstage)
-    HIPCHK(hipHostMalloc((void **)&hstage, sizeof(double) * (GRAM_SMALL + 2 * GRAM_UPLOAD),
-                         hipHostMallocMapped | hipHostMallocCoherent));
-  D.hbuf = hstage;
+  D.hbuf = stage_acquire();
+  if (!D.hbuf) return fail(COSMO_ENOMEM, "pinned staging buffer: hipHostMalloc failed");
   HIPCHK(hipHostGetDevicePointer((void **)&D.hdev, D.hbuf, 0));
   HIPCHK(hipMemsetAsync(D.B, 0, sizeof(double) * n * D.lim, D.st));
   HIPCHK(hipMemsetAsync(D.AB, 0, sizeof(double) * n * D.lim, D.st));
@@ -689,6 +717,7 @@ static int orthonormalise(Dav &D, int p, void *Vp, int q, double ortho, bool &re
         }
       if (dcur < ortho) break;
       if (dcur > dprev) {
+        if (D.verbose > 1) fprintf(stderr, "davidson ortho: restart at pass %d overlap %.3e > %.3e\n", pass, dcur, dprev);
         restart = true;
         break;
       }
@@ -714,7 +743,12 @@ static int orthonormalise(Dav &D, int p, void *Vp, int q, double ortho, bool &re
       }
       double nrm = d > 0 ? std::sqrt(d) : 0.0;
       double orig = std::sqrt(std::max(Gv[(size_t)j * k + j], 0.0));
-      if (nrm <= 1e-14 * base[j] || nrm < drop_rel * orig) continue;
+      if (nrm <= 1e-14 * base[j] || nrm < drop_rel * orig) {
+        if (D.verbose > 1)
+          fprintf(stderr, "davidson ortho: drop vector %d remaining %.3e start %.3e projected %.3e\n", j, nrm, base[j],
+                  orig);
+        continue;
+      }
       Rm[j * k + j] = nrm;
       keep.push_back(j);
     }
@@ -862,6 +896,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   D.lim = p.lim;
   D.nume = p.nume;
   D.mblock = std::min(p.mblock, p.nume);
+  D.verbose = p.verbose;
   int rc = dav_alloc(D, mixed);
   if (rc) return rc;
   const int nume = p.nume, lim = p.lim;
@@ -984,6 +1019,7 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
       }
     }
     if (done || kpass >= n) {
+      if (p.verbose > 1) fprintf(stderr, "davidson: converged by eigenvalue/coefficient test at it %d\n", nloops);
       omin_to = true;
       if (!fp64_phase) {  // FP32 phase finished: hand over to the FP64 refinement
         rc = switch_fp64();
@@ -1054,6 +1090,11 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
         }
         break;
       }
+      if (p.verbose > 1) {
+        fprintf(stderr, "davidson it %d kpass %d candidates", nloops, kpass);
+        for (int j = 0; j < ncand; ++j) fprintf(stderr, " %d:%.2e", incv[j] - 1, rnorm[incv[j] - 1]);
+        fprintf(stderr, "\n");
+      }
       nncv = nadded;
       for (int a0 = 0; a0 < nadded; a0 += 16) {
         ColList L;
@@ -1090,11 +1131,13 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
         ++restarts;
         continue;
       } else {
+        if (p.verbose > 1) fprintf(stderr, "davidson: second restart in a row at it %d (ierr 1024)\n", nloops);
         ierr += 1024;
         omin_to = true;
         break;
       }
       if (nncv == 0) {
+        if (p.verbose > 1) fprintf(stderr, "davidson: every candidate dependent at it %d\n", nloops);
         // every candidate was linearly dependent on the basis: nothing to add
         omin_to = true;
         if (!fp64_phase) {
@@ -1158,6 +1201,11 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
         icv[i] = 0;
         reopen = true;
       }
+  if (p.verbose > 1) {
+    fprintf(stderr, "davidson: exit check it %d verify %d ierr %d residuals", nloops, verify, ierr);
+    for (int i = 0; i < nume; ++i) fprintf(stderr, " %.2e", rnorm[i]);
+    fprintf(stderr, "%s\n", reopen ? " (reopened)" : "");
+  }
   if (!reopen) break;
   std::fill(s.begin(), s.end(), 0.0);
   for (int i = 0; i < nume; ++i) s[(size_t)i * (i + 1) / 2 + i] = eigs[i];
@@ -1188,6 +1236,20 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
     stats->ms_rr = D.ms_rr;
   }
   if (nloops >= p.maxiter && !omin_to) return fail(COSMO_ECONV, "Davidson reached maxiter");
+  if (p.strict) {
+    // every exit (orthogonalisation failure, every candidate dependent, the reopen cap)
+    // reports the pairs whose final residual is not below critr
+    int bad = 0;
+    double worst = 0.0;
+    for (int i = 0; i < nume; ++i)
+      if (!(rnorm[i] <= p.critr)) {
+        ++bad;
+        worst = std::max(worst, rnorm[i]);
+      }
+    if (bad)
+      return fail(COSMO_ECONV, "Davidson stopped with " + std::to_string(bad) + " pair(s) above critr (worst residual " +
+                                   std::to_string(worst) + ", ierr " + std::to_string(ierr) + ")");
+  }
   return COSMO_OK;
 }
 

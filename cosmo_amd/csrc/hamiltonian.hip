@@ -36,16 +36,30 @@ __constant__ uint32_t c_binom[33][33];
 // Device and stream are per host thread: independent solves driven from several threads
 // (batches, the server's workers) each run on their own default-priority stream and
 // overlap on the device.
+// A thread's stream comes from a per-device pool and goes back to it (drained) when the
+// thread releases it; library streams are never destroyed, so a stream handle recorded
+// by an operator always names the same stream, and short-lived worker threads (one set
+// per batch call) do not create and destroy streams each time.
 static std::mutex g_mu;
 static thread_local int t_device = 0;
 static thread_local hipStream_t t_stream[64] = {};
 static bool g_binom_done[64] = {};
+static std::vector<hipStream_t> g_stream_pool[64];
 
 hipStream_t lib_stream() {
   int d = t_device;
   if (!t_stream[d]) {
-    hipSetDevice(d);
-    hipStreamCreateWithFlags(&t_stream[d], hipStreamNonBlocking);
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      if (!g_stream_pool[d].empty()) {
+        t_stream[d] = g_stream_pool[d].back();
+        g_stream_pool[d].pop_back();
+      }
+    }
+    if (!t_stream[d]) {
+      hipSetDevice(d);
+      hipStreamCreateWithFlags(&t_stream[d], hipStreamNonBlocking);
+    }
   }
   return t_stream[d];
 }
@@ -77,7 +91,8 @@ void release_thread_streams() {
     if (t_stream[d]) {
       hipSetDevice(d);
       hipStreamSynchronize(t_stream[d]);
-      hipStreamDestroy(t_stream[d]);
+      std::lock_guard<std::mutex> lk(g_mu);
+      g_stream_pool[d].push_back(t_stream[d]);
       t_stream[d] = nullptr;
     }
 }
@@ -598,6 +613,31 @@ __global__ __launch_bounds__(64) void k_virtual_apply(DevTables T, int64_t n, co
   }
 }
 
+// number of rows whose stored entries include the diagonal (columns ascending: binary
+// search per row); one integer atomic per block
+__global__ __launch_bounds__(256) void k_count_diag(int64_t n, const int64_t *__restrict__ rowptr,
+                                                    const uint32_t *__restrict__ cols,
+                                                    unsigned long long *__restrict__ count) {
+  __shared__ unsigned int part[256];
+  unsigned int c = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = rowptr[r], hi = rowptr[r + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (cols[mid] < (uint32_t)r) lo = mid + 1;
+      else hi = mid;
+    }
+    c += (lo < rowptr[r + 1] && cols[lo] == (uint32_t)r);
+  }
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)part[0]);
+}
+
 __global__ void k_pack_cast(int64_t n, int m, const double *__restrict__ src, int lds, float *__restrict__ dst,
                             int ldd) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -823,8 +863,19 @@ int build_stored(Op &op, const System &s, int flags, hipStream_t st) {
   HIPCHK(hipStreamSynchronize(st));
   dfree(counts);
   dfree(err);
-  // upper-triangle count: every row holds its diagonal, off-diagonals come in pairs
-  M.nnz_upper = (nnz + n) / 2;
+  // upper-triangle count: off-diagonal entries come in pairs, plus the diagonal entries
+  // actually stored (a row whose terms give no diagonal element stores none)
+  unsigned long long *nd = nullptr;
+  HIPCHK(dmalloc(&nd, sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(nd, 0, sizeof(unsigned long long), st));
+  hipLaunchKernelGGL(k_count_diag, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, st, n,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, nd);
+  HIPCHK(hipGetLastError());
+  unsigned long long ndiag = 0;
+  HIPCHK(hipMemcpyAsync(&ndiag, nd, sizeof(ndiag), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(nd);
+  M.nnz_upper = (nnz + (int64_t)ndiag) / 2;
   return COSMO_OK;
 }
 

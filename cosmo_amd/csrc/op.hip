@@ -246,6 +246,12 @@ int cosmo_op_create_upper(int64_t n, const int64_t *rowptr, const int64_t *cols,
                           cosmo_op *out) {
   if (!out || n < 0 || (n && (!rowptr || !cols || !vals))) return fail(COSMO_EINVAL, "cosmo_op_create_upper: bad args");
   if (n >= (int64_t)0xffffffffLL) return fail(COSMO_EUNSUP, "dimension beyond 2^32-1");
+  // the row offsets bound every read of cols / vals, on the host and in the kernels that
+  // assemble the full rows: checked before anything is allocated or uploaded
+  if (n && rowptr[0] != 0) return fail(COSMO_EINVAL, "cosmo_op_create_upper: rowptr[0] must be 0");
+  for (int64_t r = 0; r < n; ++r)
+    if (rowptr[r + 1] < rowptr[r]) return fail(COSMO_EINVAL, "cosmo_op_create_upper: rowptr decreases at row " +
+                                                              std::to_string(r));
   auto *op = new cosmo_op_s();
   op->device = current_device();
   op->n = n;

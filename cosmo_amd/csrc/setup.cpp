@@ -433,6 +433,96 @@ static void jm_convert(const std::vector<Orbital> &o, const std::map<JKey, doubl
   }
 }
 
+// scale * J-J+ (isospin: T-T+) added to ESP and VSPX as <matrix JJ=".." TT=".."> does
+// (include/xsm/xReadSPInt.cxx:274-305):
+//  * the raising operator's one-body coefficients, keyed [annihilate, create]: J+ moves m
+//    to m+1 inside a level and species, sqrt((2j-2m)(2j+2m+2))/2; T+ moves a proton to the
+//    neutron orbital of equal n, l, j, m and parity, with T = 1 (SPJPlus / SPTPlus,
+//    include/sm/spherical2sp.cxx:623-699);
+//  * its square, every (i, j) pair of raising terms in key order: the one-body part where
+//    the created orbitals agree, and the two-body key (create_j, annihilate_i, annihilate_j,
+//    create_i) with both pairs sorted, each swap flipping the sign, products below 1e-8
+//    skipped (SPSqrM, include/sm/cioperators.cxx:200-240);
+//  * keys whose created pair precedes the annihilated one removed in map order, where the
+//    entry after a removal at the front is not examined again (StripHermitianOperator,
+//    include/sm/cioperators.cxx:274-287);
+//  * every value times scale, then added key by key into ESP and VSPX (STensor *= and +=,
+//    include/tav/stensor.cxx:259-307).
+// Two-body keys that annihilate or create one orbital twice never act on a state; they are
+// dropped from VSPX after the merge.
+template <size_t R>
+static void strip_backward(std::vector<std::pair<std::array<uint16_t, R>, double>> &v) {
+  const size_t na = R / 2;
+  auto backward = [&](const std::array<uint16_t, R> &k) {
+    for (size_t i = 0; i < na; ++i)
+      if (k[na + i] != k[i]) return k[na + i] < k[i];
+    return false;
+  };
+  size_t i = 0;
+  while (i < v.size()) {
+    if (backward(v[i].first)) {
+      v.erase(v.begin() + i);
+      i = 1;
+    } else
+      ++i;
+  }
+}
+
+static void add_raising_square(const std::vector<Orbital> &o, bool isospin, double scale, std::vector<uint16_t> &ek,
+                               std::vector<double> &ev, std::vector<uint16_t> &vk, std::vector<double> &vv) {
+  const int ns = (int)o.size();
+  std::map<std::array<uint16_t, 2>, double> jo;
+  for (int p0 = 0; p0 < ns; ++p0)
+    for (int p1 = p0 + 1; p1 < ns; ++p1) {
+      const Orbital &a = o[p0], &b = o[p1];
+      int qa, qb, top;  // raised projection (2m or 2tz) and its total (2j or 2t)
+      if (!isospin) {
+        if (a.level != b.level || a.tz2 != b.tz2) continue;
+        qa = a.m2, qb = b.m2, top = a.j2;
+      } else {
+        if (a.m2 != b.m2 || a.n != b.n || a.par != b.par || a.l != b.l || a.j2 != b.j2) continue;
+        qa = a.tz2, qb = b.tz2, top = 1;
+      }
+      if (qa + 2 == qb) jo[{(uint16_t)p0, (uint16_t)p1}] = std::sqrt((top - qa) * (top + qa + 2.)) / 2.0;
+      else if (qa - 2 == qb) jo[{(uint16_t)p1, (uint16_t)p0}] = std::sqrt((top - qb) * (top + qb + 2.)) / 2.0;
+    }
+  std::map<std::array<uint16_t, 2>, double> jsp;
+  std::map<std::array<uint16_t, 4>, double> vsp;
+  for (auto &it : jo)
+    for (auto &jt : jo) {
+      double h = it.second * jt.second;
+      if (std::fabs(h) < 1e-8) continue;
+      if (jt.first[1] == it.first[1]) jsp[{jt.first[0], it.first[0]}] += h;
+      std::array<uint16_t, 4> aa = {jt.first[1], it.first[0], jt.first[0], it.first[1]};
+      if (aa[0] > aa[1]) std::swap(aa[0], aa[1]), h = -h;
+      if (aa[2] > aa[3]) std::swap(aa[2], aa[3]), h = -h;
+      vsp[aa] += h;
+    }
+  std::vector<std::pair<std::array<uint16_t, 2>, double>> j1(jsp.begin(), jsp.end());
+  std::vector<std::pair<std::array<uint16_t, 4>, double>> j2(vsp.begin(), vsp.end());
+  strip_backward(j1);
+  strip_backward(j2);
+  std::map<std::array<uint16_t, 2>, double> e;
+  for (size_t t = 0; t < ev.size(); ++t) e[{ek[2 * t], ek[2 * t + 1]}] = ev[t];
+  for (auto &kv : j1) e[kv.first] += kv.second * scale;
+  std::map<std::array<uint16_t, 4>, double> v;
+  for (size_t t = 0; t < vv.size(); ++t) v[{vk[4 * t], vk[4 * t + 1], vk[4 * t + 2], vk[4 * t + 3]}] = vv[t];
+  for (auto &kv : j2) v[kv.first] += kv.second * scale;
+  ek.clear();
+  ev.clear();
+  for (auto &kv : e) {
+    ek.insert(ek.end(), kv.first.begin(), kv.first.end());
+    ev.push_back(kv.second);
+  }
+  vk.clear();
+  vv.clear();
+  for (auto &kv : v) {
+    if (kv.first[0] == kv.first[1] || kv.first[2] == kv.first[3]) continue;
+    vk.insert(vk.end(), kv.first.begin(), kv.first.end());
+    vv.push_back(kv.second);
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // basis structure.  The reference basis is the lexicographic order of sorted occupation
 // lists (neutrons numbered first), which is (neutron list, then proton list) order.
@@ -505,6 +595,27 @@ void System::build_basis() {
     noff.push_back(noff.back() + (psec_start[s + 1] - psec_start[s]));
   }
   dim = noff.back();
+}
+
+// value `name` of a node as ReadXMLData reads it (include/xml/xmlParser.cxx:156-177): the
+// attribute, else the text of a child element of that name; present with text that does
+// not parse as a number gives 0 (a stream extraction that fails leaves the 0 in place).
+// False when the node is absent or has neither.
+static bool matrix_term(XNode *node, const char *name, double &x) {
+  x = 0.0;
+  if (!node) return false;
+  const char *t = node->get(name);
+  std::string txt;
+  if (t) txt = t;
+  else if (XNode *c = node->child(name)) {
+    for (auto &k : c->kids)
+      if (k.name.empty()) txt += k.text;
+  } else
+    return false;
+  char *end = nullptr;
+  const double v = strtod(txt.c_str(), &end);
+  if (end != txt.c_str()) x = v;
+  return true;
 }
 
 static std::string dir_of(const std::string &p) {
@@ -678,6 +789,17 @@ int cosmo_system_from_xml(const char *xml_path, const char *sysname, cosmo_syste
   // interaction: <hamiltonian><file><name>..</name><scale rank=..>x</scale></file>
   XNode *ham = root->child("hamiltonian");
   if (!ham) return fail(COSMO_EIO, "no <hamiltonian>");
+  // what xReadSPInt would add beyond the Oxbash files and JJ/TT (include/xsm/xReadSPInt.cxx:
+  // 61-77 JISP, 106-270 tag files, 306-372 Lawson, Coulomb, Delta) is refused rather than
+  // ignored, so no input builds a different H than the reference's without an error
+  XNode *mat = sysn->child("matrix");
+  for (const char *k : {"jispfile", "tagfile"})
+    if (ham->count(k)) return fail(COSMO_EUNSUP, std::string("<hamiltonian><") + k + "> interactions are not supported");
+  for (const char *k : {"Lawson", "Coulomb", "Delta"}) {
+    double x;
+    if (matrix_term(mat, k, x) || matrix_term(ham, k, x))
+      return fail(COSMO_EUNSUP, std::string("the ") + k + " Hamiltonian term is not supported");
+  }
   std::vector<double> spe;
   std::map<JKey, double> jv;
   std::string xdir = dir_of(xml_path);
@@ -725,6 +847,12 @@ int cosmo_system_from_xml(const char *xml_path, const char *sysname, cosmo_syste
     s->esp_v.push_back(l < (int)spe.size() ? 0.0 + spe[l] : 0.0);
   }
   jm_convert(s->orb, jv, s->vspx_k, s->vspx_v);
+  // <system><matrix> add-ons, in the reference's order: JJ, then TT
+  for (int t = 0; t < 2; ++t) {
+    double scale = 0.0;
+    if (matrix_term(mat, t ? "TT" : "JJ", scale))
+      add_raising_square(s->orb, t == 1, scale, s->esp_k, s->esp_v, s->vspx_k, s->vspx_v);
+  }
   s->build_basis();
   *out = s;
   return COSMO_OK;

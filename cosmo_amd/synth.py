@@ -91,8 +91,11 @@ def _jstr(twoj: int) -> str:
 
 
 def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
-               intfile: str, mbs: str | None = None, matrix: str | None = None) -> str:
-    """CoSMo system XML (layout of include/xsm/schema.h:6-62)."""
+               intfile: str, mbs: str | None = None, matrix: str | None = None,
+               matrix_attrs: dict | None = None) -> str:
+    """CoSMo system XML (layout of include/xsm/schema.h:6-62).  matrix_attrs adds
+    attributes such as JJ / TT to the system's <matrix> node (written when either
+    matrix or matrix_attrs is given)."""
     core, cn, cz, acore, _, _ = CORES[shell]
     av = nv + zv
     jz = Fraction(twojz, 2)
@@ -103,8 +106,9 @@ def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
     extra = ""
     if mbs:
         extra += '\t\t<?ManyBodyStates name="%s" ?>\n' % mbs
-    if matrix:
-        extra += '\t\t<matrix name="%s" format="pp"/>\n' % matrix
+    if matrix or matrix_attrs:
+        attrs = "".join(' %s="%s"' % kv for kv in (matrix_attrs or {}).items())
+        extra += '\t\t<matrix name="%s" format="pp"%s/>\n' % (matrix or name + ".HH", attrs)
     return (
         '<?xml version="1.0" encoding="ISO-8859-1"?>\n<cosmo>\n'
         '\t<valencespace name="%s">\n\t\t<core name="%s" N="%d" Z="%d"/>\n%s\n\t</valencespace>\n'
@@ -116,7 +120,7 @@ def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
 
 
 def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: int = 0,
-                 parity: str = "+", seed: int = 0) -> str:
+                 parity: str = "+", seed: int = 0, matrix_attrs: dict | None = None) -> str:
     """Write ``<name>.int`` and ``<name>.xml`` into dirpath; return the XML path."""
     os.makedirs(dirpath, exist_ok=True)
     inter = make_interaction(shell, seed)
@@ -124,5 +128,33 @@ def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: i
         f.write(inter.text())
     xml = os.path.join(dirpath, name + ".xml")
     with open(xml, "w") as f:
-        f.write(system_xml(shell, name, nv, zv, twojz, parity, name + ".int"))
+        f.write(system_xml(shell, name, nv, zv, twojz, parity, name + ".int", matrix_attrs=matrix_attrs))
     return xml
+
+
+def random_requests(ca, count: int, seed: int, p_sd: float = 1.0, max_sd: int = 6, max_fp: int = 4,
+                    dirpath: str | None = None):
+    """Seeded random solve requests (the batched and the serving workloads): shell (sd with
+    probability p_sd, else fp), Nv and Zv uniform in 1..cap, 2Jz uniform over the values
+    with the parity of A in [-2A, 2A] (empty M-spaces are drawn again), the natural
+    parity of the shell, and an interaction of its own seed.  ca is the cosmo_amd module
+    (its System tells the dimension).  Returns ([(shell, nv, zv, twojz, xml, system)], rng)
+    -- the generator continues for the request's other draws."""
+    import tempfile
+    rng = np.random.default_rng(seed)
+    d = dirpath or tempfile.mkdtemp(prefix="cosmo_req_")
+    out = []
+    while len(out) < count:
+        shell = "sd" if rng.random() < p_sd else "fp"
+        cap = max_sd if shell == "sd" else max_fp
+        nv, zv = int(rng.integers(1, cap + 1)), int(rng.integers(1, cap + 1))
+        A = nv + zv
+        jmax = 2 * A
+        twojz = int(rng.choice([j for j in range(-jmax, jmax + 1) if (j - A) % 2 == 0]))
+        par = "+" if shell == "sd" else ("-" if A % 2 else "+")
+        xml = write_system(d, shell, "r%d" % len(out), nv, zv, twojz, par, int(rng.integers(1 << 30)))
+        s = ca.System.from_xml(xml)
+        if s.dim < 1:
+            continue
+        out.append((shell, nv, zv, twojz, xml, s))
+    return out, rng

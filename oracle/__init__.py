@@ -5,8 +5,9 @@ package, and only as the checker.  The product library never imports it.
 
 It restates, from the reference's behaviour, the setup half of the CoSMo hot path
 (orbital table, Oxbash reader, file formats) in Python, and drives the C restatement in
-cosmo_oracle.c (liboracle.so) for the heavy parts.  The Davidson restatement in
-davidson.py follows include/kdav/fsdavidson.cxx.
+cosmo_oracle.c (liboracle.so) for the heavy parts.  Each function is pinned by a test
+against fixtures the reference's own programs wrote (tests/golden) or by a mathematical
+check; tests/test_oracle.py names the pin of each.
 """
 from __future__ import annotations
 
@@ -269,6 +270,25 @@ def hh_times(n, rowptr, cols, vals, X):
     Y = np.zeros((n, m), order="F")
     L.orc_hh_times(n, _p(rowptr), _p(cols), _p(vals), m, _p(X), _p(Y))
     return Y
+
+
+def csb_port_times(n, rowptr, cols, vals, X, br=None):
+    """The CPU-baseline form of the CSB product (orc_csb_build / orc_csb_apply: the same
+    block traversal over an n x m row-major copy of X); returns Y (n x m)."""
+    L = lib()
+    br = int(np.sqrt(1.0 * n)) + 1 if br is None else br
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    c = np.ascontiguousarray(cols, dtype=np.int64)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    Xr = np.ascontiguousarray(X, dtype=np.float64)
+    m = Xr.shape[1]
+    Yr = np.zeros((n, m))
+    h = L.orc_csb_build(n, br, _p(rp), _p(c), _p(v))
+    try:
+        L.orc_csb_apply(h, n, m, _p(Xr), _p(Yr))
+    finally:
+        L.orc_csb_free(ctypes.c_void_p(h))
+    return Yr
 
 
 def dense_from_upper(n, rowptr, cols, vals):

@@ -87,10 +87,37 @@ static int csbtimes(const char *path, int m, int reps) {
   return 0;
 }
 
+// hhtimes <file.HH> <m> <out.bin>: the reference's .HH reader and its symmetric row product
+// (SparseMatrix<double>::Read with type = 1 as davidson_vfile sets it, TimesMatrix on
+// pointer arrays, include/tav/sparse.cxx:273-291,401-445) on m deterministic vectors
+static int hhtimes(const char *path, int m, const char *out) {
+  tav::SparseMatrix<double> FHH;
+  if (!FHH.Read(path)) return 1;
+  FHH.type = 1;
+  const uint64_t n = FHH.n;
+  std::vector<double> X(n * m), Y(n * m);
+  for (int v = 0; v < m; ++v)
+    for (uint64_t i = 0; i < n; ++i) X[v * n + i] = std::sin(0.37 * double(i) + 1.3 * double(v));
+  std::vector<double *> xp(m), yp(m);
+  for (int v = 0; v < m; ++v) xp[v] = X.data() + v * n, yp[v] = Y.data() + v * n;
+  FHH.TimesMatrix(yp.data(), xp.data(), (unsigned)m);
+  FILE *f = fopen(out, "wb");
+  uint64_t mm = m;
+  fwrite(&n, 8, 1, f); fwrite(&mm, 8, 1, f);
+  fwrite(Y.data(), 8, n * m, f);
+  fclose(f);
+  printf("{\"n\": %llu, \"m\": %d}\n", (unsigned long long)n, m);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 3) { fprintf(stderr, "usage: ref_harness dumpint|csb|csbwrite|spmm|csbtimes <file> ...\n"); return 2; }
+  if (argc < 3) {
+    fprintf(stderr, "usage: ref_harness dumpint|csb|csbwrite|spmm|csbtimes|hhtimes|davtraj <file> ...\n");
+    return 2;
+  }
   std::string mode = argv[1];
   if (mode == "csbtimes") return argc == 5 ? csbtimes(argv[2], atoi(argv[3]), atoi(argv[4])) : 2;
+  if (mode == "hhtimes") return argc == 5 ? hhtimes(argv[2], atoi(argv[3]), argv[4]) : 2;
   Setup s;
   load(s, argv[2]);
   if (mode == "dumpint") {
@@ -126,6 +153,34 @@ int main(int argc, char **argv) {
     double t_dav = now_s() - t0;
     printf("{\"n\": %llu, \"nnz_upper\": %llu, \"block_rows\": %d, \"t_build\": %.6f, \"t_davidson\": %.6f}\n",
            (unsigned long long)s.st.n, (unsigned long long)nnz, (int)br, t_build, t_dav);
+    return 0;
+  }
+  if (mode == "davtraj") {
+    // davtraj <sys.xml> <bloc> <iters> <nstate>: the reference Davidson on the CSB matrix,
+    // set up as DavidsonDiagonalization sets it up (include/kdav/davidson.cxx:167-223:
+    // numemax = mblock = bloc, crite 1e-15, critc 1e-12, critr 1e-8, ortho 1e-9,
+    // maxiter max(40 bloc, 200) when 0 is given, lim = bloc + 20), then its own matrix-
+    // vector count and error flag next to every pair's eigenvalue, |dE| and residual
+    int bloc = atoi(argv[3]), iters = atoi(argv[4]), nstate = atoi(argv[5]);
+    if (bloc < nstate) bloc = nstate;
+    MyMatrixDiagonal<tav::SparseMatrixCSB<double, uint32_t, uint16_t>> *diag =
+        new MyMatrixDiagonal<tav::SparseMatrixCSB<double, uint32_t, uint16_t>>(FHH);
+    Davidson *dav = new Davidson(diag, bloc, 1);
+    dav->crite = 1e-15; dav->critc = 1e-12; dav->critr = 1e-8; dav->ortho = 1e-9;
+    dav->silence = 1;
+    if (!iters) iters = MAX(dav->GetNumeMax() * 40, 200);
+    dav->maxiter = iters;
+    dav->SetLimMax(dav->GetNumeMax() + 20);
+    dav->mblock = bloc;
+    dav->Calculate();
+    printf("{\"n\": %llu, \"bloc\": %d, \"maxiter\": %d, \"nmv\": %lld, \"ierr\": %lld, \"pairs\": [",
+           (unsigned long long)s.st.n, bloc, iters, (long long)dav->GetMatrixVectProd(), (long long)dav->GetIerr());
+    for (int i = 0; i < bloc; ++i) {
+      eig_val e;
+      dav->EigenValue(e, i);
+      printf("%s[%.17g, %.17g, %.17g]", i ? ", " : "", e.Eigenvalue, e.Eigval_differences, e.Residual);
+    }
+    printf("]}\n");
     return 0;
   }
   if (mode == "spmm") {

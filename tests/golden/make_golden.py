@@ -4,7 +4,8 @@ Run only where the CoSMo source tree is present at /root/reference (the committe
 fixtures are what the tests read elsewhere): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
 into oracle/_ref from those sources, then this script runs them on seeded synthetic
 systems and stores inputs and outputs as small data files (npz / json / text).
-``python make_golden.py csb`` only adds the CSB file keys to the existing JSON files.
+``python make_golden.py csb|hh|traj|matrix`` only adds the CSB file keys, the .HH row
+product, the Davidson counter runs or the <matrix> JJ/TT cases.
 """
 import hashlib
 import json
@@ -143,8 +144,101 @@ def main():
             shutil.rmtree(d)
 
 
+def hh_only():
+    """Adds the reference's .HH row product (SparseMatrix::TimesMatrix, type 1) on the
+    deterministic 3-vector input: hh_y in the npz for the full cases, its SHA-256 and the
+    sampled rows otherwise."""
+    for (name, shell, nv, zv, twojz, par, seed, full, bloc) in CASES:
+        d = tempfile.mkdtemp()
+        try:
+            synth.write_system(d, shell, name, nv, zv, twojz, par, seed)
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            run([os.path.join(REF, "XHH+JJ"), name + ".xml"], d)
+            run([os.path.join(REF, "ref_harness"), "hhtimes", name + ".HH", "3", "hh.bin"], d)
+            raw = open(os.path.join(d, "hh.bin"), "rb").read()
+            n, m = struct.unpack_from("<QQ", raw, 0)
+            Y = np.frombuffer(raw, np.float64, n * m, 16).reshape(m, n).T
+            npz = os.path.join(OUT, name + ".npz")
+            arrays = dict(np.load(npz))
+            jp = os.path.join(OUT, name + ".json")
+            meta = json.load(open(jp))
+            meta["hh_y_sha256"] = hashlib.sha256(np.ascontiguousarray(Y.T).tobytes()).hexdigest()
+            if full:
+                arrays["hh_y"] = Y
+            else:
+                arrays["hh_y_sample"] = Y[arrays["sample_rows"]]
+            np.savez_compressed(npz, **arrays)
+            with open(jp, "w") as f:
+                json.dump(meta, f, indent=1)
+            print(name, "hh product", meta["hh_y_sha256"][:16])
+        finally:
+            shutil.rmtree(d)
+
+
+# (case, bloc, states) of the reference Davidson runs whose counters are kept
+TRAJ = [("sd2p2n_s11", 5, 5), ("sd2p2n_s11", 10, 10), ("sd2p2n_s11", 16, 16), ("sd3p1n_m1_s7", 6, 6),
+        ("sd3p1n_m1_s7", 3, 3), ("fp2p2n_s5", 10, 10), ("fp2p2n_s5", 4, 4), ("fp3p2n_m1_s3", 8, 8),
+        ("sd4p4n_s0", 10, 10), ("sd4p4n_s0", 5, 5)]
+
+
+def traj_only():
+    """dav_traj.json: the reference Davidson (DavidsonDiagonalization's settings) on the
+    CSB matrix of golden cases: matrix-vector products, error flag, and every pair's
+    eigenvalue, |dE| and residual."""
+    cases = {c[0]: c for c in CASES}
+    out = []
+    for (name, bloc, nstate) in TRAJ:
+        (_, shell, nv, zv, twojz, par, seed, _, _) = cases[name]
+        d = tempfile.mkdtemp()
+        try:
+            synth.write_system(d, shell, name, nv, zv, twojz, par, seed)
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            r = last_json(run([os.path.join(REF, "ref_harness"), "davtraj", name + ".xml", str(bloc), "0",
+                               str(nstate)], d))
+            r["name"] = name
+            out.append(r)
+            print(name, bloc, r["nmv"], r["ierr"], max(p[2] for p in r["pairs"]))
+        finally:
+            shutil.rmtree(d)
+    with open(os.path.join(OUT, "dav_traj.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+# Hamiltonian add-ons of <system><matrix>: name, base case, attributes
+MATRIX_CASES = [("sd2p2n_s11_jjtt", "sd2p2n_s11", {"JJ": "0.5", "TT": "0.3"}),
+                ("fp2p2n_s5_jj", "fp2p2n_s5", {"JJ": "-0.25"}),
+                ("sd3p1n_m1_s7_tt", "sd3p1n_m1_s7", {"TT": "1.5"})]
+
+
+def matrix_only():
+    """<name>.xml/.int and <name>.npz for the JJ / TT cases: the reference's m-scheme
+    tensors after xReadSPInt and the .HH rows XHH+JJ writes."""
+    cases = {c[0]: c for c in CASES}
+    for (name, base, attrs) in MATRIX_CASES:
+        (_, shell, nv, zv, twojz, par, seed, _, _) = cases[base]
+        d = tempfile.mkdtemp()
+        try:
+            xml = synth.write_system(d, shell, name, nv, zv, twojz, par, seed, matrix_attrs=attrs)
+            shutil.copy(os.path.join(d, name + ".int"), os.path.join(OUT, name + ".int"))
+            shutil.copy(xml, os.path.join(OUT, name + ".xml"))
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            run([os.path.join(REF, "ref_harness"), "dumpint", name + ".xml", "dump.bin"], d)
+            (ek, ev), (vk, vv) = read_dump(os.path.join(d, "dump.bin"))
+            run([os.path.join(REF, "XHH+JJ"), name + ".xml"], d)
+            rp, cols, vals = oracle.read_hh(os.path.join(d, name + ".HH"))
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), esp_keys=ek, esp_vals=ev, vspx_keys=vk,
+                                vspx_vals=vv, hh_rowptr=rp, hh_cols=cols, hh_vals=vals)
+            with open(os.path.join(OUT, name + ".json"), "w") as f:
+                json.dump(dict(name=name, base=base, matrix_attrs=attrs, n=int(len(rp) - 1), hh_nnz=int(rp[-1]),
+                               hh_sha256=sha(os.path.join(d, name + ".HH"))), f, indent=1)
+            print(name, len(rp) - 1, rp[-1], len(vv))
+        finally:
+            shutil.rmtree(d)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["csb"]:
-        csb_only()
+    modes = {"csb": csb_only, "hh": hh_only, "traj": traj_only, "matrix": matrix_only}
+    if sys.argv[1:2] and sys.argv[1] in modes:
+        modes[sys.argv[1]]()
     else:
         main()

@@ -181,3 +181,66 @@ def test_server_runs_unstorable_h_on_the_product_form(gpu, tmp_path):
     one = gpu.System.from_xml(synth.write_system(str(tmp_path), "sd", "one", 0, 4, 0, "+", 2))
     with pytest.raises(gpu.CosmoError):
         gpu.footprint(one, gpu.OP_FACTORED)
+
+
+def test_batch_256_uneven_sd_solves_vs_oracle(gpu):
+    """The batched workload at its stated size: 256 sd-shell solves, Nv and Zv uniform in
+    1..6, 2Jz from the allowed range, each with its own seeded interaction (n 18..93,710),
+    5 pairs each, FP32 H.V with FP64 refinement to residual 1e-6, in one batch call.
+    Every eigenvalue against the oracle's (dense / Lanczos on the oracle's H,
+    tests/golden/batch256_oracle.json) within 1e-6 MeV, every residual <= 1e-6."""
+    import json
+    import os
+    from conftest import GOLDEN
+    ref = json.load(open(os.path.join(GOLDEN, "batch256_oracle.json")))
+    reqs, _ = synth.random_requests(gpu, 256, 17)
+    systems = [r[5] for r in reqs]
+    assert [s.dim for s in systems] == [x["n"] for x in ref]
+    nume = [min(5, s.dim) for s in systems]
+    b = gpu.Batch(systems)
+    ev, rs, st = b.solve(nume, critr=1e-6, precision=gpu.PREC_FP32)
+    b.close()
+    assert st["converged"]
+    worst = 0.0
+    for s, e, r, x in zip(systems, ev, rs, ref):
+        err = np.max(np.abs(e - np.array(x["evals"])))
+        worst = max(worst, err)
+        assert err <= 1e-6, (s.dim, e, x["evals"])
+        assert np.max(r) <= 1e-6
+    print("256 solves, worst |dE| vs oracle %.2e MeV" % worst)
+
+
+def test_serving_poisson_sd_fp_mixed_stream(gpu):
+    """The serving workload: Poisson arrivals of sd (70 %) and fp (30 %, at most 4+4)
+    requests, Nv, Zv and 2Jz drawn as in the batch, k uniform in 1..10 pairs, residual
+    tolerance log-uniform in [1e-6, 1e-4], served in mixed precision while others run.
+    Each result equals the same request solved alone (same precision and parameters) and
+    lies within 1 keV of an all-FP64 solve alone; every residual meets its tolerance."""
+    reqs, rng = synth.random_requests(gpu, 24, 23, p_sd=0.7, max_sd=6, max_fp=4)
+    ks = [int(min(rng.integers(1, 11), r[5].dim)) for r in reqs]
+    tols = [float(10 ** rng.uniform(-6, -4)) for _ in reqs]
+    gaps = rng.exponential(1.0 / 40.0, size=len(reqs))  # arrivals faster than most solves finish
+    assert any(r[0] == "fp" for r in reqs) and any(r[0] == "sd" for r in reqs)
+    srv = gpu.Server(hbm_budget=96 << 30, max_active=8, precision=gpu.PREC_FP32)
+    ids = []
+    for r, k, tol, gap in zip(reqs, ks, tols, gaps):
+        time.sleep(float(gap))
+        ids.append(srv.submit(r[5], k, tol))
+    res = srv.drain()
+    srv.close()
+    byid = {x["id"]: x for x in res}
+    overlapped = 0
+    for i, r, k, tol in zip(ids, reqs, ks, tols):
+        x = byid[i]
+        assert x["status"] == 0
+        assert np.max(x["resid"]) <= tol
+        overlapped += any(y["t_admit"] < x["t_done"] and x["t_admit"] < y["t_done"] for j, y in byid.items() if j != i)
+        s = r[5]
+        lim = batch_lim(k, s.dim)
+        op = gpu.Operator.build(s)
+        e32 = gpu.davidson(op, k, want_vectors=False, critr=tol, lim=lim, maxiter=4000, precision=gpu.PREC_FP32)[0]
+        e64 = gpu.davidson(op, k, want_vectors=False, critr=tol, lim=lim, maxiter=4000)[0]
+        op.close()
+        assert np.array_equal(x["evals"], e32), "served result differs from the request solved alone"
+        assert np.max(np.abs(x["evals"] - e64)) <= 1e-3
+    assert overlapped >= len(ids) // 2  # requests really ran beside one another

@@ -94,3 +94,23 @@ def test_virtual_diagonal_equals_stored(gpu, golden):
     a = gpu.Operator.build(s)
     b = gpu.Operator.build(s, gpu.OP_VIRTUAL)
     assert np.array_equal(a.diagonal(), b.diagonal())
+
+
+@pytest.mark.parametrize("name", ["sd2p2n_s11_jjtt", "fp2p2n_s5_jj", "sd3p1n_m1_s7_tt"])
+def test_jj_tt_rows_match_reference_hh(gpu, golden, name):
+    """H with <matrix JJ / TT> add-ons: the rows XHH+JJ wrote (the reference's own .HH), same
+    structure and values to the CG noise, on the stored and the product-form operator."""
+    g = golden(name)
+    s = gpu.System.from_xml(g.xml)
+    op = gpu.Operator.build(s, gpu.OP_KEEP_SMALL)
+    rp, c, v = op.upper_rows()
+    a = g.arrays
+    assert np.array_equal(rp, a["hh_rowptr"]) and np.array_equal(c, a["hh_cols"])
+    assert np.max(np.abs(v - a["hh_vals"])) < 1e-12
+    # the reference's symmetric row product on its rows vs both operator forms
+    n = op.n
+    X = np.sin(0.37 * np.arange(n)[:, None] + 1.3 * np.arange(3)[None, :])
+    Yr = oracle.hh_times(n, a["hh_rowptr"], a["hh_cols"], a["hh_vals"], X)
+    for flags in (gpu.OP_STORED, gpu.OP_FACTORED):
+        Y = gpu.Operator.build(s, flags).apply(X)
+        assert np.max(np.abs(Y - Yr)) <= 1e-12 * np.max(np.abs(Yr))

@@ -231,3 +231,55 @@ def test_oracle_h_gives_the_reference_csb_structure(golden, name):
         f.flush()
         _, _, isha = oracle.read_csb(f.name)
     assert isha == g.meta["csb_index_sha256"]
+
+
+def _sin_x(n, m=3):
+    """The deterministic input of the reference runs: x[v][i] = sin(0.37 i + 1.3 v)."""
+    i = np.arange(n, dtype=np.float64)[:, None]
+    v = np.arange(m, dtype=np.float64)[None, :]
+    return np.sin(0.37 * i + 1.3 * v)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_hh_product_bit_exact_vs_reference(golden, name):
+    """oracle hh_times (orc_hh_times) reproduces SparseMatrix<double>::TimesMatrix with type 1
+    (include/tav/sparse.cxx:273-291, run by oracle/_ref/ref_harness hhtimes on the .HH the
+    reference's XHH+JJ wrote) bit for bit: on the reference's own rows where the fixture
+    holds them, on the oracle's rows (their SHA-256 pins every element) otherwise."""
+    g = golden(name)
+    n = g.meta["n"]
+    a = g.arrays
+    X = _sin_x(n)
+    if "spmm_x" in a:  # the same doubles as the reference's input
+        assert np.array_equal(X.view(np.uint64), a["spmm_x"].view(np.uint64))
+    if "hh_y" in a:
+        Y = oracle.hh_times(n, a["hh_rowptr"], a["hh_cols"], a["hh_vals"], X)
+        assert np.array_equal(Y.view(np.uint64), a["hh_y"].view(np.uint64))
+        assert hashlib.sha256(np.ascontiguousarray(Y.T).tobytes()).hexdigest() == g.meta["hh_y_sha256"]
+    rp, c, v = oracle.System(g.xml).h_upper(drop_small=False)
+    Y = oracle.hh_times(n, rp, c, v, X)
+    ref, got = (a["hh_y"], Y) if "hh_y" in a else (a["hh_y_sample"], Y[a["sample_rows"]])
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))  # the oracle's values: CG noise
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_csb_port_vs_reference(golden, name):
+    """The CPU-baseline port of the CSB product (orc_csb_build / orc_csb_apply, row-major
+    vectors) returns SparseMatrixCSB::TimesMatrix's output bit for bit on the reference's
+    rows (full cases), and agrees with it to 1e-12 relative on the oracle's rows."""
+    g = golden(name)
+    n = g.meta["n"]
+    a = g.arrays
+    X = _sin_x(n)
+    if "hh_rowptr" in a:
+        rp, c, v = _filtered_upper(g)
+        Y = oracle.csb_port_times(n, rp, c, v, X)
+        assert np.array_equal(Y.view(np.uint64), a["spmm_y"].view(np.uint64))
+    rp, c, v = oracle.System(g.xml).h_upper(drop_small=True)
+    Y = oracle.csb_port_times(n, rp, c, v, X)
+    ref = a["spmm_y"] if "spmm_y" in a else a["spmm_y_sample"]
+    got = Y if "spmm_y" in a else Y[a["sample_rows"]]
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # and the port equals the pinned csb_times on the same rows, bit for bit
+    Yc = oracle.csb_times(n, rp, c, v, X)
+    assert np.array_equal(Y.view(np.uint64), np.ascontiguousarray(Yc).view(np.uint64))

@@ -85,3 +85,74 @@ def test_missing_interaction_file(tmp_path):
     with pytest.raises(cosmo_amd.CosmoError) as e:
         cosmo_amd.System.from_xml(xml)
     assert e.value.code == cosmo_amd.EIO
+
+
+MATRIX_CASES = ["sd2p2n_s11_jjtt", "fp2p2n_s5_jj", "sd3p1n_m1_s7_tt"]
+
+
+@pytest.mark.parametrize("name", MATRIX_CASES)
+def test_jj_tt_tensors_match_reference(golden, name):
+    """<matrix JJ=".." TT=".."> adds scale * J-J+ / T-T+ to ESP and VSPX as the reference's
+    xReadSPInt does: ESP bit-exact and the VSPX keys exact against its dump, values within
+    the CG noise; the reference's two-body keys that annihilate or create one orbital
+    twice (they never act on a state) are the only ones this build leaves out."""
+    g = golden(name)
+    s = cosmo_amd.System.from_xml(g.xml)
+    ek, ev, vk, vv = s.tensors()
+    a = g.arrays
+    assert np.array_equal(ek, a["esp_keys"]) and np.array_equal(ev, a["esp_vals"])
+    rk, rv = a["vspx_keys"], a["vspx_vals"]
+    acts = (rk[:, 0] != rk[:, 1]) & (rk[:, 2] != rk[:, 3])
+    assert np.array_equal(vk, rk[acts])
+    assert np.max(np.abs(vv - rv[acts])) < 5e-14
+    base = golden(g.meta["base"])
+    # the add-on changes the operator: the one-body part or the two-body values differ
+    assert not (np.array_equal(ev, base.arrays["esp_vals"]) and np.array_equal(vk, base.arrays["vspx_keys"])
+                and np.max(np.abs(vv - base.arrays["vspx_vals"])) < 1e-12)
+
+
+def test_jj_shifts_only_by_j_squared(tmp_path):
+    """H + c J-J+ has the eigenvectors of H, each level moved by c (J(J+1) - M(M+1)): at M = 0
+    the shifts are c J(J+1) with J integral, so every shifted level minus an unshifted one
+    is a multiple of c that is J(J+1) for some J (the oracle's dense H on both tensors)."""
+    from cosmo_amd import synth
+    ev = {}
+    for tag, attrs in (("a", None), ("b", {"JJ": "0.75"})):
+        xml = synth.write_system(str(tmp_path), "sd", "jj" + tag, 1, 2, 1, "+", 5, matrix_attrs=attrs)
+        s = cosmo_amd.System.from_xml(xml)
+        ek, evv, vk, vv = s.tensors()
+        o = oracle.System(xml)
+        o.esp_keys, o.esp_vals, o.vspx_keys, o.vspx_vals = (np.ascontiguousarray(ek), np.ascontiguousarray(evv),
+                                                            np.ascontiguousarray(vk), np.ascontiguousarray(vv))
+        b = o.basis()
+        rp, c, v = o.h_upper(b, drop_small=False)
+        H = oracle.dense_from_upper(len(b), rp, c, v)
+        ev[tag] = (np.linalg.eigh(H), H)
+    (w0, V0), H0 = ev["a"]
+    (w1, V1), H1 = ev["b"]
+    # the added operator in the eigenbasis of H: diagonal, c (J(J+1) - M(M+1)) per level
+    D = V0.T @ (H1 - H0) @ V0
+    allowed = [0.75 * (j2 * (j2 + 2) - 3) / 4.0 for j2 in range(1, 16, 2)]  # 2J odd, 2M = 1
+    for d in np.diag(D):
+        assert min(abs(d - a) for a in allowed) < 1e-9, d
+    assert np.max(np.abs(D - np.diag(np.diag(D)))) < 1e-9
+    assert np.max(np.abs(H0 @ H1 - H1 @ H0)) < 1e-12  # [H, J-J+] = 0
+
+
+@pytest.mark.parametrize("where,snippet", [
+    ("matrix", ' Lawson="1.0"'), ("matrix", ' Coulomb="1.44"'), ("matrix", ' Delta="0.5"'),
+    ("ham", '<Lawson>2</Lawson>'), ("ham", '<tagfile fileName="x"/>'), ("ham", '<jispfile name="x"/>')])
+def test_unsupported_hamiltonian_terms_are_refused(tmp_path, where, snippet):
+    """Terms of xReadSPInt this build does not construct fail with COSMO_EUNSUP instead of
+    being ignored (the H would silently differ from the reference's)."""
+    from cosmo_amd import synth
+    xml = synth.write_system(str(tmp_path), "sd", "u", 1, 1, 0, "+", 2, matrix_attrs={"JJ": "0"})
+    text = open(xml).read()
+    if where == "matrix":
+        text = text.replace('format="pp" JJ="0"', 'format="pp"' + snippet)
+    else:
+        text = text.replace("</hamiltonian>", snippet + "</hamiltonian>")
+    open(xml, "w").write(text)
+    with pytest.raises(cosmo_amd.CosmoError) as e:
+        cosmo_amd.System.from_xml(xml)
+    assert e.value.code == cosmo_amd.EUNSUP
