@@ -102,6 +102,7 @@ SIGNATURES = {
     "cosmo_op_write_hh": (_I, [_P, _S]),
     "cosmo_op_write_csb": (_I, [_P, _S, _I]),
     "cosmo_op_timings": (_I, [_P, _P, _P]),
+    "cosmo_op_set_layout": (_I, [_P, _I]),
     "cosmo_op_dense_eigen": (_I, [_P, _P, _P]),
     "cosmo_dav_defaults": (_I, [_P, _I]),
     "cosmo_davidson": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -334,6 +335,10 @@ class Operator:
         V = np.zeros((self.n, self.n)) if vectors else None
         check(lib().cosmo_op_dense_eigen(self.h, _p(w), _p(V)))
         return w, (V.T if vectors else None)
+
+    def set_layout(self, layout):
+        """Stored product kernel: 0 automatic, 1 rows, 2 tiled."""
+        check(lib().cosmo_op_set_layout(self.h, int(layout)))
 
     def timings(self):
         b, a = ctypes.c_double(), ctypes.c_double()

@@ -944,6 +944,8 @@ int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int pr
   static bool env_done = false;
   if (!env_done) {
     if (const char *e = getenv("COSMO_SPMM_REMAP")) g_spmm_remap = atoi(e);
+    if (const char *e = getenv("COSMO_SELL")) g_sell_off = atoi(e) == 0;
+    if (const char *e = getenv("COSMO_SELL_CHUNK")) g_sell_chunk = std::max(1, atoi(e));
     env_done = true;
   }
   if (op.fact && prec != COSMO_PREC_FP64_ORDERED && ldx == ldy) {
@@ -962,6 +964,10 @@ int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int pr
       if (rc) return rc;
     }
     return COSMO_OK;
+  }
+  if (ldx == ldy && m <= ldx && prec != COSMO_PREC_FP64_ORDERED) {
+    const int rc = sell_apply(op, X, Y, m, ldx, prec, st);
+    if (rc != COSMO_EUNSUP) return rc;
   }
   const CsrDev &M = op.csr;
   for (int v0 = 0; v0 < m; v0 += 64) {
@@ -1075,7 +1081,8 @@ int stored_footprint(const System &s, int flags, int64_t *bytes) {
   dfree(err);
   int64_t nnz = 0;
   for (int64_t r = 0; r < n; ++r) nnz += hc[r];
-  *bytes = (int64_t)tab.bytes + 8 * (n + 1) + 8 * n + nnz * (4 + 8 + ((flags & COSMO_OP_NO_FP32) ? 0 : 4));
+  *bytes = (int64_t)tab.bytes + 8 * (n + 1) + 8 * n + nnz * (4 + 8 + ((flags & COSMO_OP_NO_FP32) ? 0 : 4)) +
+           sell_footprint_bound(n, nnz);
   free_tables(tab);
   if (e != hipSuccess) return fail(COSMO_EHIP, hipGetErrorString(e));
   return COSMO_OK;

@@ -1,5 +1,7 @@
 // op.h -- the device-resident operator object behind cosmo_op.
 #pragma once
+#include <vector>
+
 #include "device.h"
 
 namespace cosmo {
@@ -26,8 +28,11 @@ struct Op {
   void *sx = nullptr, *sy = nullptr, *sx32 = nullptr, *sy32 = nullptr;
   size_t s_elems = 0;
   void *fact = nullptr;        // product-form tables (COSMO_OP_FACTORED)
+  void *sell = nullptr;        // tiled layout of the stored rows (built on the first apply)
+  std::vector<int64_t> part;   // row partition: first row of each neutron configuration
   hipStream_t home = nullptr;  // library stream of the creating thread
   bool foreign = false;        // used on another stream: destroy synchronises the device
+  int layout = 0;              // stored product: 0 automatic, 1 rows, 2 tiled (sell.hip)
   double build_ms = 0, last_apply_ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -44,6 +49,12 @@ void free_factored(Op &op);
 size_t factored_bytes(const Op &op);
 int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);
 int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int prec, hipStream_t st);
+// stored H.X through the tiled layout (sell.hip); COSMO_EUNSUP where it does not apply
+int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);
+void free_sell(Op &op);
+size_t sell_bytes(const Op &op);
+int64_t sell_footprint_bound(int64_t n, int64_t nnz);
+extern int g_sell_chunk, g_sell_off;
 int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
 int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);

@@ -24,6 +24,7 @@ static void free_op(Op *op) {
   dfree(M.diag);
   free_tables(op->tab);
   free_factored(*op);
+  free_sell(*op);
   dfree(op->sx);
   dfree(op->sy);
   dfree(op->sx32);
@@ -227,6 +228,7 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
         rc = virtual_diag(op->tab.t, op->n, op->csr.diag, st);
     } else {
       rc = build_stored(*op, *s, flags, st);
+      op->part = s->noff;  // row blocks of the tiled product
     }
   }
   if (!rc) rc = fetch_diag(*op, st);
@@ -335,7 +337,7 @@ int64_t cosmo_op_bytes(cosmo_op op) {
   const CsrDev &M = op->csr;
   int64_t b = (int64_t)op->tab.bytes + 8 * M.n;
   if (!op->is_virtual) b += 8 * (M.n + 1) + M.nnz * (4 + 8 + (M.v32 ? 4 : 0));
-  return b + (int64_t)factored_bytes(*op);
+  return b + (int64_t)factored_bytes(*op) + (int64_t)sell_bytes(*op);
 }
 
 int cosmo_op_diagonal(cosmo_op op, double *d) {
@@ -362,9 +364,10 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
   hipStream_t st = lib_stream();
   note_stream(*op, st);
   const int64_t n = op->n;
-  // the product-form apply wants a power-of-two row pitch of at least 16 bytes
+  // the product-form and the tiled stored apply want a power-of-two row pitch of at least
+  // 16 bytes
   int ld = m;
-  if (op->fact) {
+  if (op->fact || (m <= 16 && precision != COSMO_PREC_FP64_ORDERED)) {
     ld = 4;
     while (ld < m) ld <<= 1;
   }
@@ -678,6 +681,12 @@ int cosmo_op_create_csb(int64_t n, int br, int bc, int nbr, int nbc, const uint3
   const int rc = csb_upper_rows(n, br, bc, nbr, nbc, get, rp, cl, vl, why);
   if (rc) return fail(rc, "cosmo_op_create_csb: " + (why.empty() ? std::string("missing block arrays") : why));
   return cosmo_op_create_upper(n, rp.data(), cl.data(), vl.data(), flags, out);
+}
+
+int cosmo_op_set_layout(cosmo_op op, int layout) {
+  if (!op || layout < 0 || layout > 2) return fail(COSMO_EINVAL, "cosmo_op_set_layout: bad args");
+  op->layout = layout;
+  return COSMO_OK;
 }
 
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms) {

@@ -1,0 +1,552 @@
+// sell.hip -- stored H.X on row blocks with the X tiles staged in LDS.
+//
+// The reference multiplies its CSB blocks one after another, each block with the slice of
+// x it needs (SparseMatrixCSB::TimesMatrix, include/tav/csbmatrix.cxx:623-663, blocks of
+// CSBBlock :128-336).  Here the blocks are the basis's own neutron-configuration blocks
+// (consecutive ones merged up to SELL_MAXROWS rows): the rows of one neutron
+// configuration, and as columns the rows of another.  Block (b, t) is non-empty only when
+// the neutron configurations of b and t are one move apart, so a row block meets a
+// hundred or so column tiles, and each entry's X row is one of the tile's rows.
+//
+// Layout (both triangles, as the full symmetric rows the builder makes): per row block b
+// and each of its column tiles t in ascending order, per slice of 64 rows (one wavefront)
+// an ELL panel: slot j of the slice's lane l is entry sl_base[q] + 64 j + l, holding the
+// column inside the tile (u16) and the value (FP32 and FP64 copies); a slice's panel is as
+// long as its longest row in t, shorter rows are padded with zero values pointing at a
+// zero row kept after each staged tile.
+// Row entries keep ascending column order, so every output element is summed over its row
+// in column order: deterministic, the same bits on every call.
+//
+// Product: one workgroup per row block, one lane per row holding its whole Y row in
+// registers.  Per column tile: the tile's X rows (contiguous in X) are loaded coalesced
+// and stored into one of two LDS buffers one tile ahead, together with the first
+// SELL_PF slots of each lane's entries, so the loads of the next tile fly while the
+// current one is multiplied out of LDS; one barrier per tile.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "op.h"
+
+namespace cosmo {
+
+#define SELL_MAXROWS 576  // rows of a row block: nine wavefronts, one lane per row
+#define SELL_PF 4         // entry slots per lane loaded one tile ahead
+#define SELL_MIN_N 65536  // smaller operators keep the row kernel
+
+struct Sell {
+  int nblk = 0, maxrows = 0, nwaves = 0, maxtiles = 0;
+  int64_t K = 0, Q = 0, E = 0, Ealloc = 0;
+  int64_t *boff = nullptr;       // [nblk+1] first row of each block
+  int32_t *nbr_start = nullptr;  // [nblk+1] into nbr_tile
+  int32_t *nbr_tile = nullptr;   // [K] column tiles of each row block, ascending
+  int64_t *qstart = nullptr;     // [nblk] first slice of the block: slice (k, w) is qstart + k * nslices + w
+  int64_t *sl_base = nullptr;    // [Q+1] first entry slot of each slice (a multiple of 64)
+  uint16_t *cols = nullptr;      // [E] column inside the tile
+  float *v32 = nullptr;          // [E]
+  double *v64 = nullptr;         // [E]
+  size_t bytes = 0;
+};
+
+struct SellView {
+  int nblk, maxrows, maxtiles, nthr;
+  const int64_t *boff;
+  const int32_t *nbr_start, *nbr_tile;
+  const int64_t *qstart, *sl_base;
+  const uint16_t *cols;
+};
+
+// ---- build -------------------------------------------------------------------------
+// column tiles met by row block b: a bitmap over tiles in LDS from the block's entries
+// (contiguous in the CSR), then either its population (pass 0) or the ascending list
+__global__ __launch_bounds__(256) void k_sell_tiles(int pass, const int64_t *__restrict__ boff,
+                                                    const int64_t *__restrict__ rowptr,
+                                                    const uint32_t *__restrict__ cols,
+                                                    const int32_t *__restrict__ blk_of, int nblk,
+                                                    int32_t *__restrict__ count,
+                                                    const int32_t *__restrict__ nbr_start,
+                                                    int32_t *__restrict__ nbr_tile) {
+  extern __shared__ uint32_t bm[];
+  const int b = blockIdx.x, nw = (nblk + 31) >> 5;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) bm[w] = 0u;
+  __syncthreads();
+  const int64_t e0 = rowptr[boff[b]], e1 = rowptr[boff[b + 1]];
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int t = blk_of[cols[e]];
+    atomicOr(&bm[t >> 5], 1u << (t & 31));
+  }
+  __syncthreads();
+  if (pass == 0) {
+    __shared__ int tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    int c = 0;
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) c += __popc(bm[w]);
+    atomicAdd(&tot, c);
+    __syncthreads();
+    if (threadIdx.x == 0) count[b] = tot;
+  } else if (threadIdx.x == 0) {
+    int o = nbr_start[b];
+    for (int w = 0; w < nw; ++w)
+      for (uint32_t m = bm[w]; m; m &= m - 1) nbr_tile[o++] = w * 32 + __ffs(m) - 1;
+  }
+}
+
+// pass 0: per slice and tile, the longest row (entries of the row in the tile), written
+// as slot counts to len[q]; pass 1: the entries into their slots.  One lane per row, every
+// lane walks the block's tiles in order with a cursor into its (column-sorted) row.
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
+    int pass, const int64_t *__restrict__ boff, const int64_t *__restrict__ rowptr,
+    const uint32_t *__restrict__ cols, const double *__restrict__ v64, const float *__restrict__ v32,
+    const int32_t *__restrict__ nbr_start, const int32_t *__restrict__ nbr_tile,
+    const int64_t *__restrict__ qstart, int64_t *__restrict__ len, const int64_t *__restrict__ sl_base,
+    uint16_t *__restrict__ scol, double *__restrict__ s64, float *__restrict__ s32) {
+  const int b = blockIdx.x;
+  const int64_t r0 = boff[b];
+  const int S = (int)(boff[b + 1] - r0), ns = (S + 63) >> 6;
+  const int rho = threadIdx.x, lane = rho & 63, w = rho >> 6;
+  if (w >= ns) return;  // no barrier below: whole idle wavefronts leave
+  const bool on = rho < S;
+  int64_t e = on ? rowptr[r0 + rho] : 0;
+  const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
+  const int k0 = nbr_start[b], k1 = nbr_start[b + 1];
+  for (int k = k0; k < k1; ++k) {
+    const int t = nbr_tile[k];
+    const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
+    const int64_t q = qstart[b] + (int64_t)(k - k0) * ns + w;
+    int c = 0;
+    if (pass == 0) {
+      while (e < ee && cols[e] < ce) ++e, ++c;
+      for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
+      if (lane == 0) len[q] = c;
+    } else {
+      const int64_t base = sl_base[q] + lane;
+      while (e < ee && cols[e] < ce) {
+        const int64_t p = base + (int64_t)c * 64;
+        if (scol) scol[p] = (uint16_t)(cols[e] - cs);
+        if (s64) s64[p] = v64[e];
+        if (s32) s32[p] = v32[e];
+        ++e;
+        ++c;
+      }
+    }
+  }
+}
+
+// ---- product -----------------------------------------------------------------------
+// 16-byte pieces as native vectors (an array of HIP's union vector types is kept in
+// scratch memory by the compiler; arrays of these stay in registers)
+template <typename T>
+struct V16t;
+template <>
+struct V16t<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct V16t<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// XCD-aware order: the eight XCDs take chunks of C consecutive row blocks in turn, so each
+// XCD works through neighbouring blocks (which share column tiles in its L2) and the XCDs
+// stay close together in the basis (sharing X in the Infinity Cache)
+__device__ __forceinline__ int sell_block(int chunk) {
+  const int G = gridDim.x, i = blockIdx.x;
+  const int G8 = (G / (8 * chunk)) * (8 * chunk);
+  if (chunk <= 1 || i >= G8) return i;
+  const int x = i % 8, j = i / 8;
+  return ((j / chunk) * 8 + x) * chunk + j % chunk;
+}
+
+template <typename T, int LD>
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals,
+                                                            const T *__restrict__ X, T *__restrict__ Y, int m,
+                                                            int chunk) {
+  typedef typename V16t<T>::type V;
+  constexpr int VPR = (int)(LD * sizeof(T) / 16);  // 16-byte pieces per X row
+  constexpr int VW = (int)(16 / sizeof(T));        // elements per piece
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int b = sell_block(chunk);
+  const int64_t r0 = A.boff[b];
+  const int S = (int)(A.boff[b + 1] - r0), ns = (S + 63) >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x, wc = min(w, ns - 1);
+  const bool wave_on = w < ns;
+  const int k0 = A.nbr_start[b], K = A.nbr_start[b + 1] - k0;
+  const int64_t qb = A.qstart[b];
+  // LDS: two tile buffers of nthr + 1 rows (row nthr stays zero: the padding slots' row;
+  // the direct loads fill rows [0, nthr)), then the block's tile table (first X row,
+  // 16-byte pieces) and its slice offsets (64-slot units from the block's first slot)
+  const int zrow = A.nthr;
+  V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(zrow + 1) * VPR;
+  uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf1 + (size_t)(zrow + 1) * VPR);
+  int32_t *mt_np = reinterpret_cast<int32_t *>(mt_c0 + A.maxtiles);
+  uint16_t *mt_off = reinterpret_cast<uint16_t *>(mt_np + A.maxtiles);
+  const int64_t slot0 = A.sl_base[qb];
+  for (int i = tid; i < K; i += nthr) {
+    const int t = A.nbr_tile[k0 + i];
+    const int64_t c0 = A.boff[t];
+    mt_c0[i] = (uint32_t)c0;
+    mt_np[i] = (int)(A.boff[t + 1] - c0) * VPR;
+  }
+  for (int i = tid; i <= K * ns; i += nthr) mt_off[i] = (uint16_t)((A.sl_base[qb + i] - slot0) >> 6);
+  if (tid < 2 * VPR) buf0[(size_t)(tid / VPR) * (zrow + 1) * VPR + (size_t)zrow * VPR + tid % VPR] = V{};
+  T acc[LD];
+#pragma unroll
+  for (int i = 0; i < LD; ++i) acc[i] = (T)0;
+  // tile loads straight into LDS (global_load_lds_dwordx4: lane i of a wavefront writes
+  // 16 bytes at the wavefront's LDS base + 16 i): piece idx = tid + p * nthr of the tile's
+  // contiguous X rows goes to LDS piece idx; pieces past the tile load the tile's last
+  // piece again (one cached line per wavefront) into rows no slot refers to
+  auto dma_tile = [&](int kk, V *dst) {
+    const int np = mt_np[kk];
+    const V *src = reinterpret_cast<const V *>(X + (int64_t)mt_c0[kk] * LD);
+#pragma unroll
+    for (int p = 0; p < VPR; ++p)
+      __builtin_amdgcn_global_load_lds((const void *)(src + min(tid + p * nthr, np - 1)),
+                                       (__attribute__((address_space(3))) void *)(dst + (w * 64 + p * nthr)), 16, 0,
+                                       0);
+  };
+  // slots [j0, j0 + SELL_PF) of this lane's panel for step kk, read unconditionally (the
+  // slot arrays carry SELL_PF * 64 slots of slack); slots past the panel's end are turned
+  // into (zero row, 0) with masks rather than selects, so the compiler keeps every load
+  // unconditional and never waits for the next tile's loads at a branch
+  auto slice_of = [&](int kk, int64_t &base, int &L) {
+    const int o = mt_off[kk * ns + wc];
+    base = slot0 + (int64_t)o * 64 + lane;
+    L = (int)mt_off[kk * ns + wc + 1] - o;
+  };
+  auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[SELL_PF], T (&h)[SELL_PF]) {
+    (void)L;
+#pragma unroll
+    for (int j = 0; j < SELL_PF; ++j) {
+      const int64_t p = base + (int64_t)(j0 + j) * 64;
+      c[j] = A.cols[p];
+      h[j] = vals[p];
+    }
+  };
+  // instruction p of an entry reads piece (p + lane) % VPR of its X row into accumulator
+  // block p: lanes on random rows then hit all LDS banks (a plain piece-p read from rows
+  // 64 bytes apart reaches a quarter of them); the rotation is undone when Y is written
+  int rot[VPR];
+#pragma unroll
+  for (int p = 0; p < VPR; ++p) rot[p] = (p + lane) & (VPR - 1);
+  auto fma_row = [&](const V *tile, uint32_t c, T h) {
+    const V *xr = tile + c * VPR;
+#pragma unroll
+    for (int p = 0; p < VPR; ++p) {
+      const V u = xr[rot[p]];
+#pragma unroll
+      for (int i = 0; i < VW; ++i) acc[p * VW + i] += h * u[i];
+    }
+  };
+  __syncthreads();  // tile table in LDS
+  // One tile step: the next tile's X pieces and the first slot group of its panel are
+  // loaded into the "next" register set while the current tile is multiplied; long
+  // panels alternate between two groups (one in flight while the other is used).  The
+  // step loop is unrolled by two so the sets trade roles instead of being copied: a copy
+  // of a register whose load is in flight makes the compiler wait for every load issued
+  // before it, the next tile's included.
+  uint32_t c0s[SELL_PF], c1s[SELL_PF], cx[SELL_PF];
+  T h0s[SELL_PF], h1s[SELL_PF], hx[SELL_PF];
+  int64_t base0 = 0, base1 = 0;
+  int L0 = 0, L1 = 0;
+  // slots j0 + j past the panel's end L act as (zero row, 0), applied here at the use and
+  // not at the load, so the loads stay in flight until the group is reached
+  auto group = [&](const V *cur, const uint32_t (&c)[SELL_PF], const T (&h)[SELL_PF], int j0, int L) {
+#pragma unroll
+    for (int j = 0; j < SELL_PF; ++j) {
+      const bool in = j0 + j < L;
+      fma_row(cur, in ? c[j] : (uint32_t)zrow, in ? h[j] : (T)0);
+    }
+  };
+  auto step = [&](int kk, const V *cur, V *nxt, uint32_t (&ca)[SELL_PF], T (&ha)[SELL_PF], int64_t base_a, int La,
+                  uint32_t (&cb)[SELL_PF], T (&hb)[SELL_PF], int64_t &base_b, int &Lb) {
+    // the next step's loads are issued unconditionally (the last step reloads its own tile
+    // and slots, unused): a load under a branch would make the first use after the join
+    // wait for every load in flight, the next tile's included
+    const bool more = kk + 1 < K;
+    const int kn = more ? kk + 1 : kk;
+    slice_of(kn, base_b, Lb);
+    load_slots(base_b, Lb, 0, cb, hb);
+    if (more) dma_tile(kn, nxt);  // the buffer read in the previous step: free since its barrier
+    if (wave_on && La > 0) {
+      group(cur, ca, ha, 0, La);
+      if (SELL_PF < La) {
+        load_slots(base_a, La, SELL_PF, cx, hx);
+        for (int j0 = SELL_PF; j0 < La; j0 += 2 * SELL_PF) {
+          if (j0 + SELL_PF < La) load_slots(base_a, La, j0 + SELL_PF, ca, ha);
+          group(cur, cx, hx, j0, La);
+          if (j0 + SELL_PF >= La) break;
+          if (j0 + 2 * SELL_PF < La) load_slots(base_a, La, j0 + 2 * SELL_PF, cx, hx);
+          group(cur, ca, ha, j0 + SELL_PF, La);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wavefront's direct loads have landed
+    __syncthreads();
+  };
+  if (K > 0) {
+    slice_of(0, base0, L0);
+    load_slots(base0, L0, 0, c0s, h0s);
+    dma_tile(0, buf0);
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    __syncthreads();
+    for (int kk = 0; kk < K; kk += 2) {
+      step(kk, buf0, buf1, c0s, h0s, base0, L0, c1s, h1s, base1, L1);
+      if (kk + 1 < K) step(kk + 1, buf1, buf0, c1s, h1s, base1, L1, c0s, h0s, base0, L0);
+    }
+  }
+  if (tid < S) {
+    T *y = Y + (r0 + tid) * LD;
+#pragma unroll
+    for (int p = 0; p < VPR; ++p) {
+      // accumulator block p holds piece rot[p] of the row
+      if ((rot[p] + 1) * VW <= m) {
+        V u;
+#pragma unroll
+        for (int i = 0; i < VW; ++i) u[i] = acc[p * VW + i];
+        reinterpret_cast<V *>(y)[rot[p]] = u;
+      } else {
+#pragma unroll
+        for (int i = 0; i < VW; ++i)
+          if (rot[p] * VW + i < m) y[rot[p] * VW + i] = acc[p * VW + i];
+      }
+    }
+  }
+}
+
+// ---- host side ------------------------------------------------------------------------
+static std::mutex g_sell_mu;
+
+void free_sell(Op &op) {
+  Sell *s = (Sell *)op.sell;
+  if (!s) return;
+  for (void *p : {(void *)s->boff, (void *)s->nbr_start, (void *)s->nbr_tile, (void *)s->qstart,
+                  (void *)s->sl_base, (void *)s->cols, (void *)s->v32, (void *)s->v64})
+    dfree(p);
+  delete s;
+  op.sell = nullptr;
+}
+
+size_t sell_bytes(const Op &op) { return op.sell ? ((const Sell *)op.sell)->bytes : 0; }
+
+// row blocks: the operator's row partition (neutron-configuration blocks) with
+// consecutive blocks merged while they fit SELL_MAXROWS rows and longer ones split; an
+// operator without a partition gets blocks of SELL_MAXROWS rows
+static std::vector<int64_t> sell_partition(const Op &op) {
+  const int64_t n = op.n;
+  std::vector<int64_t> p = op.part;
+  if (p.size() < 2 || p.front() != 0 || p.back() != n) {
+    p.clear();
+    for (int64_t r = 0; r < n; r += SELL_MAXROWS) p.push_back(r);
+    p.push_back(n);
+  }
+  std::vector<int64_t> out{0};  // out.back(): first row of the block being formed
+  for (size_t i = 0; i + 1 < p.size(); ++i) {
+    const int64_t a = p[i], e = p[i + 1];
+    if (e - a > SELL_MAXROWS) {  // an oversized block: equal pieces of its own
+      if (a > out.back()) out.push_back(a);
+      const int64_t pieces = (e - a + SELL_MAXROWS - 1) / SELL_MAXROWS, step = (e - a + pieces - 1) / pieces;
+      for (int64_t x = a + step; x < e; x += step) out.push_back(x);
+      out.push_back(e);
+      continue;
+    }
+    if (e - out.back() > SELL_MAXROWS) out.push_back(a);  // [a, e) starts a new block
+  }
+  if (out.back() != n) out.push_back(n);
+  return out;
+}
+
+// LDS bytes of the per-block tile table and slice offsets
+static size_t sell_meta_bytes(const Sell &s) {
+  return 8 * (size_t)s.maxtiles + 2 * ((size_t)s.maxtiles * s.nwaves + 1) + 16;
+}
+
+static int build_sell(Op &op, hipStream_t st) {
+  const CsrDev &M = op.csr;
+  const int64_t n = op.n;
+  std::vector<int64_t> hb = sell_partition(op);
+  const int nblk = (int)hb.size() - 1;
+  if (nblk <= 0) return fail(COSMO_EINVAL, "empty operator");
+  if (nblk > (1 << 17)) return fail(COSMO_EUNSUP, "too many row blocks for the tiled product");
+  int maxrows = 0;
+  for (int b = 0; b < nblk; ++b) maxrows = std::max<int>(maxrows, (int)(hb[b + 1] - hb[b]));
+  auto *s = new Sell();
+  s->nblk = nblk;
+  s->maxrows = maxrows;
+  s->nwaves = (maxrows + 63) / 64;
+  auto bail = [&](int rc) {
+    op.sell = s;
+    free_sell(op);
+    return rc;
+  };
+  std::vector<int32_t> hblk(n);
+  for (int b = 0; b < nblk; ++b)
+    for (int64_t r = hb[b]; r < hb[b + 1]; ++r) hblk[r] = b;
+  int32_t *blk_of = nullptr, *cnt = nullptr;
+  int64_t *len = nullptr;
+  if (dmalloc(&s->boff, sizeof(int64_t) * (nblk + 1)) || dmalloc(&blk_of, sizeof(int32_t) * n) ||
+      dmalloc(&cnt, sizeof(int32_t) * nblk) || dmalloc(&s->nbr_start, sizeof(int32_t) * (nblk + 1)))
+    return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
+  HIPCHK(hipMemcpyAsync(s->boff, hb.data(), sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(blk_of, hblk.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  const size_t bm_bytes = sizeof(uint32_t) * (size_t)((nblk + 31) / 32);
+  hipLaunchKernelGGL(k_sell_tiles, dim3(nblk), dim3(256), bm_bytes, st, 0, (const int64_t *)s->boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)blk_of, nblk, cnt,
+                     (const int32_t *)nullptr, (int32_t *)nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<int32_t> hc(nblk), hs(nblk + 1, 0);
+  HIPCHK(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < nblk; ++b) hs[b + 1] = hs[b] + hc[b];
+  s->K = hs[nblk];
+  std::vector<int64_t> hq(nblk);
+  int64_t Q = 0;
+  for (int b = 0; b < nblk; ++b) {
+    hq[b] = Q;
+    Q += (int64_t)hc[b] * ((hb[b + 1] - hb[b] + 63) / 64);
+  }
+  s->Q = Q;
+  if (dmalloc(&s->nbr_tile, sizeof(int32_t) * std::max<int64_t>(s->K, 1)) ||
+      dmalloc(&s->qstart, sizeof(int64_t) * nblk) || dmalloc(&len, sizeof(int64_t) * std::max<int64_t>(Q, 1)) ||
+      dmalloc(&s->sl_base, sizeof(int64_t) * (Q + 1)))
+    return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
+  HIPCHK(hipMemcpyAsync(s->nbr_start, hs.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->qstart, hq.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_sell_tiles, dim3(nblk), dim3(256), bm_bytes, st, 1, (const int64_t *)s->boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)blk_of, nblk, cnt,
+                     (const int32_t *)s->nbr_start, s->nbr_tile);
+  HIPCHK(hipGetLastError());
+  const int thr = s->nwaves * 64;
+  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 0, (const int64_t *)s->boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
+                     (const int64_t *)s->qstart, len, (const int64_t *)nullptr, (uint16_t *)nullptr,
+                     (double *)nullptr, (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<int64_t> hl(Q), hbase(Q + 1, 0);
+  HIPCHK(hipMemcpyAsync(hl.data(), len, sizeof(int64_t) * Q, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int64_t q = 0; q < Q; ++q) hbase[q + 1] = hbase[q] + 64 * hl[q];
+  s->E = hbase[Q];
+  // the product keeps each block's slice offsets in LDS as 16-bit counts of 64 slots
+  for (int b = 0; b < nblk; ++b) {
+    s->maxtiles = std::max(s->maxtiles, hc[b]);
+    const int64_t qe = hq[b] + (int64_t)hc[b] * ((hb[b + 1] - hb[b] + 63) / 64);
+    if ((hbase[qe] - hbase[hq[b]]) / 64 > 65535)
+      return bail(fail(COSMO_EUNSUP, "a row block too long for the tiled product"));
+  }
+
+  // slack: a lane reads up to SELL_PF slots past its slice's panel
+  const int64_t E = s->E + 64 * SELL_PF;
+  if (dmalloc(&s->cols, sizeof(uint16_t) * E)) return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
+  s->Ealloc = E;
+  HIPCHK(hipMemcpyAsync(s->sl_base, hbase.data(), sizeof(int64_t) * (Q + 1), hipMemcpyHostToDevice, st));
+  // padding slots point at the zero row kept after every staged tile (index nwaves * 64)
+  HIPCHK(hipMemsetD16Async((hipDeviceptr_t)s->cols, (unsigned short)(s->nwaves * 64), E, st));
+  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 1, (const int64_t *)s->boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
+                     (const int64_t *)s->qstart, (int64_t *)nullptr, (const int64_t *)s->sl_base, s->cols,
+                     (double *)nullptr, (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(blk_of);
+  dfree(cnt);
+  dfree(len);
+  s->bytes = sizeof(int64_t) * (nblk + 1) * 2 + sizeof(int32_t) * (nblk + 1 + s->K) + sizeof(int64_t) * (Q + 1) +
+             (size_t)E * 2;
+  op.sell = s;
+  return COSMO_OK;
+}
+
+// the slot values of one precision, filled on its first product (the FP32 and the FP64
+// copies are only kept for the precisions in use)
+static int sell_values(Op &op, bool f32, hipStream_t st) {
+  Sell &s = *(Sell *)op.sell;
+  if (f32 ? s.v32 != nullptr : s.v64 != nullptr) return COSMO_OK;
+  const CsrDev &M = op.csr;
+  void *dst = nullptr;
+  if (dmalloc_void(&dst, (f32 ? sizeof(float) : sizeof(double)) * s.Ealloc, st))
+    return fail(COSMO_ENOMEM, "tiled layout: value allocation failed");
+  HIPCHK(hipMemsetAsync(dst, 0, (f32 ? sizeof(float) : sizeof(double)) * s.Ealloc, st));
+  hipLaunchKernelGGL(k_sell_slots, dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1, (const int64_t *)s.boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)M.v64,
+                     (const float *)M.v32, (const int32_t *)s.nbr_start, (const int32_t *)s.nbr_tile,
+                     (const int64_t *)s.qstart, (int64_t *)nullptr, (const int64_t *)s.sl_base, (uint16_t *)nullptr,
+                     f32 ? (double *)nullptr : (double *)dst, f32 ? (float *)dst : (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  if (f32) s.v32 = (float *)dst;
+  else s.v64 = (double *)dst;
+  s.bytes += (f32 ? sizeof(float) : sizeof(double)) * (size_t)s.Ealloc;
+  return COSMO_OK;
+}
+
+// device bytes the tiled layout of an operator with n rows and nnz stored entries can
+// take with both value copies (the padding of the panels stays below 1.7 x on the
+// systems measured): the server's admission counts it beside the rows
+int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
+  if (n < SELL_MIN_N) return 0;
+  return (int64_t)(1.7 * (double)nnz) * (2 + 4 + 8) + 64 * n;
+}
+
+int g_sell_chunk = 8;  // XCD chunk of row blocks (COSMO_SELL_CHUNK)
+int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
+
+template <typename T, int LD>
+static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {
+  SellView v{s.nblk, s.maxrows, s.maxtiles, s.nwaves * 64, s.boff, s.nbr_start, s.nbr_tile, s.qstart, s.sl_base,
+             s.cols};
+  const size_t lds = 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(s);
+  if (lds > 160 * 1024) return COSMO_EUNSUP;  // tiles and table do not fit one CU's LDS
+  auto kern = k_sell_apply<T, LD>;
+  static bool attr_done = false;
+  if (!attr_done) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_done = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, g_sell_chunk);
+  HIPCHK(hipGetLastError());
+  return COSMO_OK;
+}
+
+// Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
+// 16-byte X rows up to 16 elements); COSMO_EUNSUP otherwise (the caller falls back).
+int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st) {
+  if (op.is_virtual || !op.csr.rowptr || op.layout == 1) return COSMO_EUNSUP;
+  if (op.layout == 0 && (g_sell_off || op.n < SELL_MIN_N)) return COSMO_EUNSUP;
+  const bool f32 = prec == COSMO_PREC_FP32;
+  if (!(f32 || prec == COSMO_PREC_FP64)) return COSMO_EUNSUP;
+  if (f32 && !(ld == 4 || ld == 8 || ld == 16)) return COSMO_EUNSUP;
+  if (!f32 && !(ld == 2 || ld == 4 || ld == 8 || ld == 16)) return COSMO_EUNSUP;
+  if (f32 && !op.csr.v32) return COSMO_EUNSUP;
+  {
+    std::lock_guard<std::mutex> lk(g_sell_mu);
+    if (!op.sell) {
+      const int rc = build_sell(op, st);
+      if (rc) return rc;
+    }
+    const int rc = sell_values(op, f32, st);
+    if (rc) return rc;
+  }
+  const Sell &s = *(const Sell *)op.sell;
+  if (f32) {
+    const float *x = (const float *)X;
+    float *y = (float *)Y;
+    if (ld == 4) return launch_sell<float, 4>(s, s.v32, x, y, m, st);
+    if (ld == 8) return launch_sell<float, 8>(s, s.v32, x, y, m, st);
+    return launch_sell<float, 16>(s, s.v32, x, y, m, st);
+  }
+  const double *x = (const double *)X;
+  double *y = (double *)Y;
+  if (ld == 2) return launch_sell<double, 2>(s, s.v64, x, y, m, st);
+  if (ld == 4) return launch_sell<double, 4>(s, s.v64, x, y, m, st);
+  if (ld == 8) return launch_sell<double, 8>(s, s.v64, x, y, m, st);
+  return launch_sell<double, 16>(s, s.v64, x, y, m, st);
+}
+
+}  // namespace cosmo
