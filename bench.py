@@ -30,7 +30,8 @@ Reported beside it:
     row sample of the same matrix.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-batch]
-                       [--no-serve] [--no-cpu] [--calibrate-copy] [--profile-solve fp64|mixed]
+                       [--no-serve] [--no-cpu] [--layout auto|rows|tiled] [--calibrate-copy]
+                       [--profile-solve fp64|mixed]
 """
 import argparse
 import json
@@ -80,8 +81,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-virtual", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline row sample (0: n/8)")
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_spmm_vec_fp4p4n_k16_traffic.json"),
-                    help="HBM bytes per launch of the stored kernel from rocprofv3 counters")
+    ap.add_argument("--layout", choices=["auto", "rows", "tiled"], default="auto",
+                    help="stored-H kernel: auto (tiled from n = 65536), rows (one wave per row) or tiled")
+    ap.add_argument("--traffic-json", default=None,
+                    help="HBM bytes per launch of the stored kernel from rocprofv3 counters "
+                         "(default: the committed summary for this kernel and workload, if any)")
     ap.add_argument("--profile-solve", choices=["fp64", "mixed"], default=None,
                     help="only one 10-pair product-form solve (block 16, basis limit 100) in this precision, "
                          "after one warm-up solve; for kernel profiles")
@@ -132,6 +136,8 @@ def main():
     print("[bench] system %s n=%d setup %.2fs" % (name, n, t_setup), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     op = ca.Operator.build(sysm)
+    if args.layout != "auto":
+        op.set_layout({"rows": 1, "tiled": 2}[args.layout])
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     nnz_u, nnz_f = op.nnz()
@@ -164,9 +170,16 @@ def main():
         ref = hr @ Xh
         maxrel = max(maxrel, float(np.max(np.abs(Yh[r] - ref)) / max(1e-30, np.max(np.abs(ref)))))
 
+    # X and Y are dense n x k, so the row pitch is k: the tiled kernel serves pitches 4, 8, 16
+    tiled = k in (4, 8, 16) and (args.layout == "tiled" or (args.layout == "auto" and n >= 65536))
+    kname = "k_sell_apply<float,%d>" % k if tiled else "k_spmm_vec<float,4>"
+    kfile = ("k_sell_apply" if tiled else "k_spmm_vec") + "_%s_k%d_traffic.json" % (name, k)
     traffic = None
-    if args.traffic_json and os.path.exists(args.traffic_json):
-        traffic = json.load(open(args.traffic_json)).get("hbm_bytes_per_launch")
+    tj = args.traffic_json or os.path.join(ROOT, "profiles", kfile)
+    if os.path.exists(tj) and (args.traffic_json or args.seed == 2):
+        tr = json.load(open(tj))
+        if tr.get("kernel") == kname:
+            traffic = tr.get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -182,7 +195,7 @@ def main():
                "build_s": round(t_build, 3), "setup_s": round(t_setup, 3), "device_bytes": op.bytes()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmm_vec<float,4>", "bytes_per_launch": bytes_alg},
+                     "kernel": kname, "bytes_per_launch": bytes_alg},
     }
 
     print("[bench] spot check of %d rows: max rel err %.2e" % (len(rows), maxrel), file=sys.stderr, flush=True)

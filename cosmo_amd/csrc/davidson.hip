@@ -338,7 +338,7 @@ struct Dav {
   int es = 8;
   void *Bv = nullptr, *ABv = nullptr, *Rv = nullptr, *Tv = nullptr;
   double *partial = nullptr, *segs = nullptr, *small = nullptr;
-  // pinned host staging (per host thread, kept): GRAM_SMALL doubles of Gram read-back,
+  // pinned host staging (one buffer of the library-wide pool per running solve): GRAM_SMALL doubles of Gram read-back,
   // then two upload slots of GRAM_UPLOAD doubles used alternately
   double *hbuf = nullptr, *hdev = nullptr;  // hdev: the device's address of hbuf
   bool up_busy[2] = {false, false};

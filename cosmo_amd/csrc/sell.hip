@@ -18,10 +18,11 @@
 // in column order: deterministic, the same bits on every call.
 //
 // Product: one workgroup per row block, one lane per row holding its whole Y row in
-// registers.  Per column tile: the tile's X rows (contiguous in X) are loaded coalesced
-// and stored into one of two LDS buffers one tile ahead, together with the first
-// SELL_PF slots of each lane's entries, so the loads of the next tile fly while the
-// current one is multiplied out of LDS; one barrier per tile.
+// registers.  Per column tile: the tile's X rows (contiguous in X) go from memory straight
+// into one of two LDS buffers (global_load_lds, no registers involved) one tile ahead, and
+// the first SELL_PF slots of each lane's next panel are loaded into registers, so the
+// loads of the next tile fly while the current one is multiplied out of LDS; one wait and
+// one barrier per tile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -134,6 +135,36 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
       }
     }
   }
+}
+
+// layout statistics: the distinct tile rows the (row block, column tile) pairs reference,
+// summed into *out (one workgroup per row block, a bitmap of the tile's rows in LDS)
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *__restrict__ boff,
+                                                               const int64_t *__restrict__ rowptr,
+                                                               const uint32_t *__restrict__ cols,
+                                                               const int32_t *__restrict__ nbr_start,
+                                                               const int32_t *__restrict__ nbr_tile,
+                                                               unsigned long long *__restrict__ out) {
+  constexpr int NW = SELL_MAXROWS / 32;
+  __shared__ uint32_t bm[NW];
+  __shared__ int tot;
+  const int b = blockIdx.x, rho = threadIdx.x;
+  const int64_t r0 = boff[b];
+  const bool on = rho < (int)(boff[b + 1] - r0);
+  int64_t e = on ? rowptr[r0 + rho] : 0;
+  const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
+  if (rho == 0) tot = 0;
+  for (int k = nbr_start[b]; k < nbr_start[b + 1]; ++k) {
+    if (rho < NW) bm[rho] = 0u;
+    __syncthreads();
+    const int t = nbr_tile[k];
+    const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
+    for (; e < ee && cols[e] < ce; ++e) atomicOr(&bm[(cols[e] - cs) >> 5], 1u << ((cols[e] - cs) & 31));
+    __syncthreads();
+    if (rho < NW) atomicAdd(&tot, __popc(bm[rho]));
+    __syncthreads();
+  }
+  if (rho == 0) atomicAdd(out, (unsigned long long)tot);
 }
 
 // ---- product -----------------------------------------------------------------------
@@ -459,6 +490,45 @@ static int build_sell(Op &op, hipStream_t st) {
   dfree(len);
   s->bytes = sizeof(int64_t) * (nblk + 1) * 2 + sizeof(int32_t) * (nblk + 1 + s->K) + sizeof(int64_t) * (Q + 1) +
              (size_t)E * 2;
+  if (g_sell_stats) {
+    // X rows one product stages: every (row block, column tile) pair loads the tile's rows
+    std::vector<int32_t> ht(std::max<int64_t>(s->K, 1));
+    HIPCHK(hipMemcpy(ht.data(), s->nbr_tile, sizeof(int32_t) * s->K, hipMemcpyDeviceToHost));
+    int64_t staged = 0;
+    for (int64_t t = 0; t < s->K; ++t) staged += hb[ht[t] + 1] - hb[ht[t]];
+    // the same if g neighbouring row blocks staged the union of their tiles once
+    for (int g : {2, 4, 8}) {
+      int64_t sg = 0;
+      std::vector<char> seen(nblk, 0);
+      std::vector<int32_t> list;
+      for (int b0 = 0; b0 < nblk; b0 += g) {
+        list.clear();
+        for (int b = b0; b < std::min(nblk, b0 + g); ++b)
+          for (int k = hs[b]; k < hs[b + 1]; ++k)
+            if (!seen[ht[k]]) seen[ht[k]] = 1, list.push_back(ht[k]);
+        for (int32_t t : list) sg += hb[t + 1] - hb[t], seen[t] = 0;
+      }
+      fprintf(stderr, "[sell] groups of %d blocks: staged X rows %lld (%.3f n)\n", g, (long long)sg,
+              (double)sg / (double)n);
+    }
+    unsigned long long *dd = nullptr, distinct = 0;
+    if (!dmalloc(&dd, sizeof(unsigned long long))) {
+      HIPCHK(hipMemsetAsync(dd, 0, sizeof(unsigned long long), st));
+      hipLaunchKernelGGL(k_sell_distinct, dim3(nblk), dim3(thr), 0, st, (const int64_t *)s->boff,
+                         (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)s->nbr_start,
+                         (const int32_t *)s->nbr_tile, dd);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&distinct, dd, sizeof(distinct), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      dfree(dd);
+    }
+    fprintf(stderr,
+            "[sell] n=%lld blocks=%d (rows <= %d) tiles=%lld slices=%lld slots=%lld stored=%lld "
+            "padding=%.4f staged X rows=%lld (%.3f n) referenced=%llu (%.3f n)\n",
+            (long long)n, nblk, maxrows, (long long)s->K, (long long)Q, (long long)s->E, (long long)M.nnz,
+            M.nnz ? (double)s->E / (double)M.nnz : 0.0, (long long)staged, (double)staged / (double)n, distinct,
+            (double)distinct / (double)n);
+  }
   op.sell = s;
   return COSMO_OK;
 }
@@ -496,6 +566,7 @@ int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
 
 int g_sell_chunk = 8;  // XCD chunk of row blocks (COSMO_SELL_CHUNK)
 int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
+int g_sell_stats = 0;  // COSMO_SELL_STATS=1: the layout's sizes on stderr when it is built
 
 template <typename T, int LD>
 static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {

@@ -283,3 +283,34 @@ def test_oracle_csb_port_vs_reference(golden, name):
     # and the port equals the pinned csb_times on the same rows, bit for bit
     Yc = oracle.csb_times(n, rp, c, v, X)
     assert np.array_equal(Y.view(np.uint64), np.ascontiguousarray(Yc).view(np.uint64))
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_mbs_reader_on_the_reference_bytes(golden, name, tmp_path):
+    """read_mbs on a file whose SHA-256 is the reference XsysmbsOMP output's returns the
+    oracle's basis (the file is written by the library's .mbs writer, which needs no GPU)."""
+    import cosmo_amd
+    g = golden(name)
+    p = str(tmp_path / "b.mbs")
+    cosmo_amd.System.from_xml(g.xml).write_mbs(p)
+    assert hashlib.sha256(open(p, "rb").read()).hexdigest() == g.meta["mbs_sha256"]
+    assert np.array_equal(oracle.read_mbs(p), oracle.System(g.xml).basis())
+
+
+def test_oracle_ee_reader_round_trip(tmp_path):
+    """read_ee against the library's .EE writer: counts, values, residuals and vectors bit
+    for bit; a file cut short fails to parse."""
+    import cosmo_amd
+    rng = np.random.default_rng(4)
+    n, k = 37, 3
+    E, dE, R = rng.standard_normal(k), rng.standard_normal(k) * 1e-9, rng.random(k) * 1e-7
+    V = rng.standard_normal((k, n))
+    p = str(tmp_path / "x.EE")
+    cosmo_amd.write_ee(p, E, dE, R, V.T)  # vectors as columns, n x k
+    e2, d2, r2, v2 = oracle.read_ee(p)
+    assert np.array_equal(e2, E) and np.array_equal(d2, dE) and np.array_equal(r2, R)
+    assert all(np.array_equal(a, b) for a, b in zip(v2, V))
+    assert os.path.getsize(p) == 4 + k * (32 + 8 * n)
+    open(p, "r+b").truncate(4 + 32 + 8 * n + 16)
+    with pytest.raises(Exception):
+        oracle.read_ee(p)
