@@ -98,6 +98,7 @@ SIGNATURES = {
     "cosmo_op_diagonal": (_I, [_P, _P]),
     "cosmo_op_apply": (_I, [_P, _P, _P, _I, _I]),
     "cosmo_op_apply_device": (_I, [_P, _P, _P, _I, _I, _I, _P]),
+    "cosmo_op_apply_device_noalloc": (_I, [_P, _P, _P, _I, _I, _I, _P]),
     "cosmo_op_upper_rows": (_I, [_P, _L, _L, _P, _P, _P, _P]),
     "cosmo_op_write_hh": (_I, [_P, _S]),
     "cosmo_op_write_csb": (_I, [_P, _S, _I]),
@@ -311,6 +312,14 @@ class Operator:
         s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
         check(lib().cosmo_op_apply_device(self.h, ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(Y.data_ptr()), m, ld,
                                           precision, s))
+
+    def apply_device_noalloc(self, X, Y, precision=PREC_FP64, stream=None):
+        """apply_device allocating nothing (product-form operators; the scratch is a fixed
+        64 MiB device array per GPU, shared by every call on it)."""
+        m, ld = X.shape[1], X.stride(0)
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        check(lib().cosmo_op_apply_device_noalloc(self.h, ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(Y.data_ptr()),
+                                                  m, ld, precision, s))
 
     def upper_rows(self, r0=0, r1=None):
         r1 = self.n if r1 is None else r1

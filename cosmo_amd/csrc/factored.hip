@@ -33,6 +33,8 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
+#include <string>
 #include <tuple>
 #include <memory>
 #include <vector>
@@ -1077,20 +1079,24 @@ static int launch_fact(const FactOp &f, int lpe, const T *X, const T *Yp, const 
   return 1;
 }
 
+// the class products of every class, Yp (neutron) and Yq (proton) in X's row order
+template <typename T>
+static int launch_class_mm(const FactOp &f, const T *X, int ld, T *Yp, T *Yq, hipStream_t st) {
+  const int li = __builtin_ctz((unsigned)ld);
+  if (li >= MM_NLD) return fail(COSMO_EUNSUP, "factorised apply: row pitch above 64");
+  if (f.mm_total[li] > 0)
+    hipLaunchKernelGGL(k_class_mm<T>, dim3((unsigned)f.mm_total[li]), dim3(256), 0, st, f.mm, f.mm_first[li],
+                       f.nprob, li, ld, X, Yp, Yq);
+  HIPCHK(hipGetLastError());
+  return COSMO_OK;
+}
+
 // Everything one apply puts on the stream (no waits, no allocation), up to the join of the
 // forked GEMMs; k_fact follows with the chosen slice width.
 template <typename T>
 static int fact_enqueue_single(FactOp &f, const T *X, int ld, hipStream_t st) {
   T *Xp = (T *)f.Xp, *Yp = (T *)f.Yp, *Xq = (T *)f.Xq, *Yq = (T *)f.Yq;
-  if (f.use_mm) {
-    const int li = __builtin_ctz((unsigned)ld);
-    if (li >= MM_NLD) return fail(COSMO_EUNSUP, "factorised apply: row pitch above 64");
-    if (f.mm_total[li] > 0)
-      hipLaunchKernelGGL(k_class_mm<T>, dim3((unsigned)f.mm_total[li]), dim3(256), 0, st, f.mm, f.mm_first[li],
-                         f.nprob, li, ld, X, Yp, Yq);
-    HIPCHK(hipGetLastError());
-    return COSMO_OK;
-  }
+  if (f.use_mm) return launch_class_mm<T>(f, X, ld, Yp, Yq, st);
   hipLaunchKernelGGL(k_class_gather<T>, dim3((unsigned)f.A.nblk), dim3(256), 0, st, f.A.nblk, f.A.noff, f.A.poff,
                      f.A.qoff, f.A.qnb, X, Xp, Xq, ld);
   HIPCHK(hipGetLastError());
@@ -1253,6 +1259,70 @@ static int fact_apply_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st)
   if (r < 0) return r;
   if (r == 0) return fail(COSMO_EUNSUP, "factorised apply: block tables exceed the LDS");
   return COSMO_OK;
+}
+
+// ---- the product without allocation -------------------------------------------------
+// For callers that cannot allocate (code under HIP graph capture, tight serving loops) the
+// class products' outputs (Yp, then Yq, each n * ld elements rounded up to 256 bytes) live
+// in a file-scope device array of FACT_SCRATCH_BYTES, one per GPU, held until the process
+// ends.  Every call on a GPU shares it, so calls that overlap on one GPU race.  Nothing in
+// it needs clearing: the class products write every element the step kernel reads.
+constexpr size_t FACT_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[FACT_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};  // per device: the array's device address
+
+// the array's address on device dev (current): looked up once per device
+static int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= 64) return fail(COSMO_EINVAL, "scratch: device index");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(COSMO_EHIP, std::string("scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return COSMO_OK;
+}
+
+template <typename T>
+static int fact_apply_noalloc_t(Op &op, const T *X, T *Y, int m, int ld, hipStream_t st) {
+  FactOp &f = *static_cast<FactOp *>(op.fact);
+  constexpr int VPL = Piece16<T>::N;
+  if (ld % VPL || (ld & (ld - 1)) || ((uintptr_t)X | (uintptr_t)Y) % 16)
+    return fail(COSMO_EUNSUP, "factorised apply: leading dimension must be a power of two of >= 16 bytes");
+  if (!f.use_mm)
+    return fail(COSMO_EUNSUP, "cosmo_op_apply_device_noalloc: this operator's class products need the allocating path");
+  const size_t half = ((size_t)f.n * ld * sizeof(T) + 255) & ~(size_t)255;
+  if (2 * half > FACT_SCRATCH_BYTES)
+    return fail(COSMO_EINVAL, "cosmo_op_apply_device_noalloc: needs " + std::to_string(2 * half) +
+                                  " bytes of scratch, above the maximum of " + std::to_string(FACT_SCRATCH_BYTES));
+  void *base = nullptr;
+  int rc = scratch_address(op.device, &base);
+  if (rc) return rc;
+  T *Yp = (T *)base, *Yq = (T *)((char *)base + half);
+  rc = launch_class_mm<T>(f, X, ld, Yp, Yq, st);
+  if (rc) return rc;
+  // the slice width tuned by the allocating path for this (precision, ld), else the widest
+  // that fits: every width gives the same bits
+  const auto it = f.tuned.find((int)sizeof(T) * 4096 + ld);
+  if (it != f.tuned.end()) {
+    const int r = launch_fact<T>(f, it->second, X, Yp, Yq, Y, m, ld, st);
+    if (r != 0) return r < 0 ? r : COSMO_OK;
+  }
+  for (int lpe = 8; lpe >= 1; lpe >>= 1) {
+    const int r = launch_fact<T>(f, lpe, X, Yp, Yq, Y, m, ld, st);
+    if (r != 0) return r < 0 ? r : COSMO_OK;
+  }
+  return fail(COSMO_EUNSUP, "factorised apply: proton sector too large for one workgroup");
+}
+
+int fact_apply_noalloc(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st) {
+  if (prec == COSMO_PREC_FP32) return fact_apply_noalloc_t<float>(op, (const float *)X, (float *)Y, m, ld, st);
+  if (prec == COSMO_PREC_FP64) return fact_apply_noalloc_t<double>(op, (const double *)X, (double *)Y, m, ld, st);
+  return fail(COSMO_EUNSUP, "cosmo_op_apply_device_noalloc: precision must be COSMO_PREC_FP32 or COSMO_PREC_FP64");
 }
 
 int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st) {

@@ -48,6 +48,7 @@ int64_t factored_footprint(const System &s);  // -1: no product form
 void free_factored(Op &op);
 size_t factored_bytes(const Op &op);
 int fact_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);
+int fact_apply_noalloc(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);
 int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int prec, hipStream_t st);
 // stored H.X through the tiled layout (sell.hip); COSMO_EUNSUP where it does not apply
 int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st);

@@ -357,6 +357,19 @@ int cosmo_op_apply_device(cosmo_op op, const void *X, void *Y, int m, int ld, in
   return rc;
 }
 
+int cosmo_op_apply_device_noalloc(cosmo_op op, const void *X, void *Y, int m, int ld, int precision, void *stream) {
+  if (!op || (!X && m) || (!Y && m) || m < 0 || ld < m)
+    return fail(COSMO_EINVAL, "cosmo_op_apply_device_noalloc: bad args");
+  if (!op->fact)
+    return fail(COSMO_EUNSUP, "cosmo_op_apply_device_noalloc: product-form operators (COSMO_OP_FACTORED) only");
+  if (m == 0) return COSMO_OK;
+  hipSetDevice(op->device);
+  // the caller's stream, else the null stream: no library stream is created here
+  hipStream_t st = (hipStream_t)stream;
+  note_stream(*op, st);
+  return fact_apply_noalloc(*op, X, Y, m, ld, precision, st);
+}
+
 int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m, int precision) {
   if (!op || m < 0 || (m && (!x || !y))) return fail(COSMO_EINVAL, "cosmo_op_apply: bad args");
   if (m == 0) return COSMO_OK;

@@ -156,6 +156,16 @@ int cosmo_op_apply(cosmo_op op, double *const *y, const double *const *x, int m,
  * stream may be NULL (the library's stream). */
 int cosmo_op_apply_device(cosmo_op op, const void *X, void *Y, int m, int ld, int precision,
                           void *stream);
+/* No counterpart in the reference.  cosmo_op_apply_device for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): the same product, with
+ * the same bits, allocating nothing (no hipMalloc/hipFree, no allocating library call, no
+ * tuning, no graph capture).  Product-form operators (COSMO_OP_FACTORED) only, FP32 or
+ * FP64 (not FP64_ORDERED), ld a power of two of >= 16 bytes.  The class products' scratch
+ * is a 64 MiB device array per GPU held until the process ends: calls needing more than
+ * 2 * n * ld * element size bytes (each half rounded up to 256) fail with COSMO_EINVAL.
+ * Calls that overlap on one GPU share that array and race.  stream NULL: the null stream. */
+int cosmo_op_apply_device_noalloc(cosmo_op op, const void *X, void *Y, int m, int ld, int precision,
+                                  void *stream);
 /* Upper-triangle rows [r0, r1) as stored (cols ascending, diagonal first): counts[r-r0]
  * always; cols/vals filled when non-NULL at offsets given by rowptr (exclusive prefix of
  * counts, relative to r0).  This is what XHH+JJ writes to .HH (PPMatrix.cxx:413-465). */
