@@ -10,18 +10,7 @@
 #include <sstream>
 
 #include "cli_common.h"
-
-static std::string spin_str(int twice) {
-  if (twice % 2 == 0) return std::to_string(twice / 2);
-  return std::to_string(twice) + "/2";
-}
-
-static std::string num(double x) {
-  std::ostringstream o;
-  o.precision(10);
-  o << x;
-  return o.str();
-}
+#include "states_xml.h"
 
 int main(int argc, char **argv) {
   Args a = parse_args(argc, argv, {{"i", "iterations"}, {"n", "number"}, {"b", "block"}, {"f", "file"},
@@ -101,73 +90,21 @@ int main(int argc, char **argv) {
     printf("\n");
   }
   if (cosmo_write_ee(eef.c_str(), n, count, ev.data(), de.data(), rs.data(), vec.data())) return die("write .EE");
-  // spin / isospin labels of the saved states
-  int lstates = count;
-  if (nstate > 0 && nstate < lstates) lstates = nstate;
-  std::vector<double> jj(lstates), tt(lstates);
-  if (cosmo_system_labels(s, vec.data(), lstates, jj.data(), tt.data())) return die("labels");
-  int nparticles, nsps, nn, np;
-  int64_t ne, nvx;
-  cosmo_system_info(s, &nparticles, &nsps, &nn, &np, &ne, &nvx);
-  int twoM = 0;
-  {
-    std::string jz = sysn->get("Jz") ? sysn->get("Jz") : "0";
-    size_t sl = jz.find('/');
-    twoM = sl == std::string::npos ? 2 * atoi(jz.c_str()) : atoi(jz.substr(0, sl).c_str());
-  }
-  int nv = sysn->get("Nv") ? atoi(sysn->get("Nv")) : 0, zv = sysn->get("Zv") ? atoi(sysn->get("Zv")) : 0;
-  int twoTz = nv - zv;
-  const char *par = sysn->get("P") ? sysn->get("P") : "+";
-  for (int i = 0; i < lstates; ++i) {
-    int J2 = cosmo_twice_spin(jj[i], twoM), T2 = cosmo_twice_spin(tt[i], twoTz);
-    fprintf(stderr, "State: %d E= %.6f JJ= %.6f TT= %.6f\n", i, ev[i], jj[i], tt[i]);
-    cosmo::XNode *st = nullptr;
-    for (auto &k : sysn->kids)
-      if (k.name == "state" && k.get("J") && k.get("T") && k.get("E") && spin_str(J2) == k.get("J") &&
-          spin_str(T2) == k.get("T") && std::fabs(atof(k.get("E")) - ev[i]) < 1e-6)
-        st = &k;
-    if (!st) {
-      cosmo::XNode k;
-      k.name = "state";
-      k.set("name", spin_str(J2));
-      k.set("J", spin_str(J2));
-      k.set("P", par[0] == '-' ? "-" : "+");
-      k.set("T", spin_str(T2));
-      k.set("E", num(ev[i]));
-      sysn->kids.push_back(k);
-      st = &sysn->kids.back();
-    }
-    bool have = false;
-    for (auto &w : st->kids)
-      if (w.name == "wavefunction" && w.get("file") && eef == w.get("file") && w.get("id") && atoi(w.get("id")) == i)
-        have = true;
-    if (!have) {
-      cosmo::XNode w;
-      w.name = "wavefunction";
-      w.set("Jz", spin_str(twoM));
-      w.set("Tz", spin_str(twoTz));
-      w.set("file", eef);
-      w.set("id", std::to_string(i));
-      st->kids.push_back(w);
-    }
-    st->set("dE", num(de[i]));
-  }
-  // states ordered by energy after the other children
-  std::stable_sort(sysn->kids.begin(), sysn->kids.end(), [](const cosmo::XNode &x, const cosmo::XNode &y) {
-    bool sx = x.name == "state", sy = y.name == "state";
-    if (sx != sy) return !sx;
-    if (!sx) return false;
-    return atof(x.get("E")) < atof(y.get("E"));
-  });
-  cosmo::XNode *mat = sysn->child("matrix");
+  // the matrix node the labelling reads (the reference expects it in the input)
+  cosmo::XNode *mat = sysn->child("matrix", -1);
   if (!mat) {
     cosmo::XNode m;
     m.name = "matrix";
     sysn->kids.insert(sysn->kids.begin(), m);
-    mat = sysn->child("matrix");
+    mat = sysn->child("matrix", -1);
   }
   mat->set("name", sysname + ".HH");
   mat->set("format", "pp");
+  // spin / isospin labels, J / T filters, JJ / TT shifts and the state nodes (XSHLJT over
+  // the saved states)
+  int lstates = count;
+  if (nstate > 0 && nstate < lstates) lstates = nstate;
+  if (label_states(s, sysn, eef, lstates, ev.data(), de.data(), vec.data())) return 1;
   std::string out = a.flags.count("o") ? a.flags["o"] : xml;
   if (!cosmo::xml_write_file(out, doc)) return fprintf(stderr, "Error: cannot write %s\n", out.c_str()), 1;
   cosmo_op_destroy(op);

@@ -197,6 +197,9 @@ extern "C" int cosmo_system_labels(cosmo_system s, const double *evecs, int coun
 
 // twice the spin from <J-J+> and twice its projection (include/sm/QN.cxx:40-55)
 extern "C" int cosmo_twice_spin(double x, int twoM) {
-  double s2 = std::sqrt(4.0 * x + (double)twoM * (twoM + 2) + 1.0) - 1.0;
+  const double s2 = std::sqrt(4.0 * x + (double)twoM * (twoM + 2) + 1.0) - 1.0;
+  // J + M must be an integer to within 0.1, else the spin is rejected as -1
+  const double jm = 0.5 * (s2 + twoM);
+  if (!(std::fabs(jm - std::round(jm)) < 0.1)) return -1;
   return (int)std::lround(s2);
 }

@@ -776,11 +776,12 @@ int cosmo_system_from_xml(const char *xml_path, const char *sysname, cosmo_syste
     zv = atoi(need("Zv"));
     if (av < 0) av = nv + zv;
   } else if (av >= 0 && need("Tz")) {
+    // valence Tz = (N - Z) / 2 (include/xsm/xManyBodyStates.cxx:47-55)
     int tz2;
-    if (parse_half(need("Tz"), tz2)) return fail(COSMO_EIO, "bad Tz");
-    int tz = tz2 / 2;
-    zv = (av - tz) / 2;
-    nv = (av + tz) / 2;
+    if (parse_half(need("Tz"), tz2) || ((av + tz2) & 1) || std::abs(tz2) > av)
+      return fail(COSMO_EIO, "bad Tz for the valence particle number");
+    zv = (av - tz2) / 2;
+    nv = (av + tz2) / 2;
   } else
     return fail(COSMO_EIO, "system needs Nv and Zv (or Av and Tz); interactive prompts are not supported");
   int twoM = av & 1;

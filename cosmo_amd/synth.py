@@ -92,17 +92,22 @@ def _jstr(twoj: int) -> str:
 
 def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
                intfile: str, mbs: str | None = None, matrix: str | None = None,
-               matrix_attrs: dict | None = None) -> str:
+               matrix_attrs: dict | None = None, system_attrs: dict | None = None) -> str:
     """CoSMo system XML (layout of include/xsm/schema.h:6-62).  matrix_attrs adds
     attributes such as JJ / TT to the system's <matrix> node (written when either
-    matrix or matrix_attrs is given)."""
+    matrix or matrix_attrs is given); system_attrs adds attributes such as the J / T
+    filters of the state labelling to the <system> node."""
     core, cn, cz, acore, _, _ = CORES[shell]
     av = nv + zv
-    jz = Fraction(twojz, 2)
-    jzs = str(jz.numerator) if jz.denominator == 1 else "%d/%d" % (jz.numerator, jz.denominator)
+    def spin_str(twice):  # a spin as the reference's XML writes it: "1", "3/2", "-1/2"
+        x = Fraction(twice, 2)
+        return str(x.numerator) if x.denominator == 1 else "%d/%d" % (x.numerator, x.denominator)
+    jzs = spin_str(twojz)
+    tzs = spin_str(nv - zv)  # valence Tz = (N - Z) / 2
     orb = "\n".join(
         '\t\t<orbital name="%s" n="%d" l="%d" j="%s" type="pn" index="%d"/>' % (o[0], o[1], o[2], _jstr(o[3]), i + 1)
         for i, o in enumerate(SHELLS[shell]))
+    sattrs = "".join(' %s="%s"' % kv for kv in (system_attrs or {}).items())
     extra = ""
     if mbs:
         extra += '\t\t<?ManyBodyStates name="%s" ?>\n' % mbs
@@ -114,13 +119,14 @@ def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
         '\t<valencespace name="%s">\n\t\t<core name="%s" N="%d" Z="%d"/>\n%s\n\t</valencespace>\n'
         '\t<hamiltonian name="synthetic">\n\t\t<file format="ascii">\n\t\t\t<name>%s</name>\n'
         '\t\t</file>\n\t</hamiltonian>\n'
-        '\t<system name="%s" A="%d" Av="%d" Nv="%d" Zv="%d" Tz="%d" Jz="%s" P="%s">\n%s\t</system>\n'
-        '</cosmo>\n' % (shell, core, cn, cz, orb, intfile, name, cn + cz + av, av, nv, zv, nv - zv, jzs,
-                        parity, extra))
+        '\t<system name="%s" A="%d" Av="%d" Nv="%d" Zv="%d" Tz="%s" Jz="%s" P="%s"%s>\n%s\t</system>\n'
+        '</cosmo>\n' % (shell, core, cn, cz, orb, intfile, name, cn + cz + av, av, nv, zv, tzs, jzs,
+                        parity, sattrs, extra))
 
 
 def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: int = 0,
-                 parity: str = "+", seed: int = 0, matrix_attrs: dict | None = None) -> str:
+                 parity: str = "+", seed: int = 0, matrix_attrs: dict | None = None,
+                 system_attrs: dict | None = None) -> str:
     """Write ``<name>.int`` and ``<name>.xml`` into dirpath; return the XML path."""
     os.makedirs(dirpath, exist_ok=True)
     inter = make_interaction(shell, seed)
@@ -128,7 +134,8 @@ def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: i
         f.write(inter.text())
     xml = os.path.join(dirpath, name + ".xml")
     with open(xml, "w") as f:
-        f.write(system_xml(shell, name, nv, zv, twojz, parity, name + ".int", matrix_attrs=matrix_attrs))
+        f.write(system_xml(shell, name, nv, zv, twojz, parity, name + ".int", matrix_attrs=matrix_attrs,
+                           system_attrs=system_attrs))
     return xml
 
 

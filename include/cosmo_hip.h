@@ -83,7 +83,8 @@ int cosmo_system_orbitals(cosmo_system s, int *level, int *j2, int *m2, int *tz2
  * Replaces PPMatrix::Sqr of the J+ / T+ operators in XSHLJT
  * (include/xsm/XSHLJT.cxx:38-142, include/sm/PPMatrix.cxx:400-411). */
 int cosmo_system_labels(cosmo_system s, const double *evecs, int count, double *jm_jp, double *tm_tp);
-/* 2J from <J-J+> and 2M: sqrt(4x + 2M(2M+2) + 1) - 1 rounded (include/sm/QN.cxx:40-55). */
+/* 2J from <J-J+> and 2M: sqrt(4x + 2M(2M+2) + 1) - 1 rounded, or -1 when J + M is not an
+ * integer to within 0.1 (include/sm/QN.cxx:40-55). */
 int cosmo_twice_spin(double x, int twoM);
 /* .mbs file I/O (include/sm/ManyBodyStates.cxx:199-246).  cosmo_system_check_mbs fails
  * with COSMO_EUNSUP when the file's basis differs from the full M-scheme basis

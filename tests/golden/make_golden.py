@@ -1,11 +1,12 @@
 """Writes the golden fixtures of tests/golden from the reference's own programs.
 
 Run only where the CoSMo source tree is present at /root/reference (the committed
-fixtures are what the tests read elsewhere): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile and ref_harness
+fixtures are what the tests read elsewhere): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile, XSHLJT and ref_harness
 into oracle/_ref from those sources, then this script runs them on seeded synthetic
 systems and stores inputs and outputs as small data files (npz / json / text).
-``python make_golden.py csb|hh|traj|matrix`` only adds the CSB file keys, the .HH row
-product, the Davidson counter runs or the <matrix> JJ/TT cases.
+``python make_golden.py csb|hh|traj|matrix|labels`` only adds the CSB file keys, the .HH row
+product, the Davidson counter runs, the <matrix> JJ/TT cases or the state labelling
+(XSHLJT) cases.
 """
 import hashlib
 import json
@@ -236,8 +237,50 @@ def matrix_only():
             shutil.rmtree(d)
 
 
+# state labelling (XSHLJT): name, base case, <matrix> attributes, <system> attributes, states
+LABEL_CASES = [("sd2p2n_s11_lab", "sd2p2n_s11", {"JJ": "0.5", "TT": "0.3"}, {}, 10),
+               ("sd3p1n_m1_s7_labj", "sd3p1n_m1_s7", {"TT": "1.5"}, {"J": "2"}, 6),
+               ("fp2p2n_s5_labt", "fp2p2n_s5", {"JJ": "-0.25"}, {"T": "1"}, 10)]
+
+
+def xml_states(path):
+    """The <state> nodes of the system, attributes in file order, with their wavefunctions."""
+    import xml.etree.ElementTree as ET
+    out = []
+    for st in ET.parse(path).getroot().find("system").findall("state"):
+        out.append(dict(attrs=list(st.attrib.items()),
+                        wavefunctions=[list(w.attrib.items()) for w in st.findall("wavefunction")]))
+    return out
+
+
+def labels_only():
+    """<name>_in.xml (the system as XSHLJT reads it: after XsysmbsOMP and XHH+JJ), <name>.int,
+    <name>_EE.bin (the .EE davidson_vfile wrote) and <name>_states.json (the <state> nodes
+    the reference XSHLJT added from it)."""
+    cases = {c[0]: c for c in CASES}
+    for (name, base, mattrs, sattrs, nst) in LABEL_CASES:
+        (_, shell, nv, zv, twojz, par, seed, _, _) = cases[base]
+        d = tempfile.mkdtemp()
+        try:
+            synth.write_system(d, shell, name, nv, zv, twojz, par, seed, matrix_attrs=mattrs, system_attrs=sattrs)
+            shutil.copy(os.path.join(d, name + ".int"), os.path.join(OUT, name + ".int"))
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            run([os.path.join(REF, "XHH+JJ"), name + ".xml"], d)
+            shutil.copy(os.path.join(d, name + ".xml"), os.path.join(OUT, name + "_in.xml"))
+            run([os.path.join(REF, "davidson_vfile"), name + ".HH", "-b", str(nst), "-n", str(nst)], d)
+            shutil.copy(os.path.join(d, name + ".HH.EE"), os.path.join(OUT, name + "_EE.bin"))
+            run([os.path.join(REF, "XSHLJT"), name + ".xml", "-n", str(nst)], d)
+            states = xml_states(os.path.join(d, name + ".xml"))
+            with open(os.path.join(OUT, name + "_states.json"), "w") as f:
+                json.dump(dict(name=name, base=base, matrix_attrs=mattrs, system_attrs=sattrs, states_asked=nst,
+                               states=states), f, indent=1)
+            print(name, len(states), [dict(x["attrs"])["name"] for x in states])
+        finally:
+            shutil.rmtree(d)
+
+
 if __name__ == "__main__":
-    modes = {"csb": csb_only, "hh": hh_only, "traj": traj_only, "matrix": matrix_only}
+    modes = {"csb": csb_only, "hh": hh_only, "traj": traj_only, "matrix": matrix_only, "labels": labels_only}
     if sys.argv[1:2] and sys.argv[1] in modes:
         modes[sys.argv[1]]()
     else:

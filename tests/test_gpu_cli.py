@@ -86,3 +86,71 @@ def test_labels_integrality(gpu, golden, name, twom, twotz):
     for x in tt:
         T2 = lib.cosmo_twice_spin(float(x), twotz)
         assert abs(x - (T2 * (T2 + 2) - twotz * (twotz + 2)) / 4.0) < 1e-6
+
+
+LABEL_CASES = ["sd2p2n_s11_lab", "sd3p1n_m1_s7_labj", "fp2p2n_s5_labt"]
+
+
+def _states(path):
+    """<state> nodes of the system: attributes in file order, with their wavefunctions."""
+    out = []
+    for st in ET.parse(path).getroot().find("system").findall("state"):
+        out.append(dict(attrs=[list(kv) for kv in st.attrib.items()],
+                        wavefunctions=[[list(kv) for kv in w.attrib.items()] for w in st.findall("wavefunction")]))
+    return out
+
+
+def _label_case(name, tmp_path):
+    import json
+    import shutil
+    from conftest import GOLDEN
+    ref = json.load(open(os.path.join(GOLDEN, name + "_states.json")))
+    shutil.copy(os.path.join(GOLDEN, name + "_in.xml"), str(tmp_path / (name + ".xml")))
+    shutil.copy(os.path.join(GOLDEN, name + ".int"), str(tmp_path / (name + ".int")))
+    return ref
+
+
+@pytest.mark.parametrize("name", LABEL_CASES)
+def test_xshljt_on_the_reference_ee(gpu, tmp_path, name):
+    """XSHLJT on the .EE the reference's davidson_vfile wrote gives the <state> nodes the
+    reference's XSHLJT wrote from it, attribute for attribute and in order: J and T labels,
+    the system's J / T filters, the matrix's JJ / TT shifts taken off the energies, names
+    J P(k), Ex, dE and the wavefunction references (tests/golden/*_states.json)."""
+    import shutil
+    from conftest import GOLDEN
+    ref = _label_case(name, tmp_path)
+    shutil.copy(os.path.join(GOLDEN, name + "_EE.bin"), str(tmp_path / (name + ".HH.EE")))
+    run("XSHLJT", name + ".xml", "-n", str(ref["states_asked"]), cwd=str(tmp_path))
+    assert _states(str(tmp_path / (name + ".xml"))) == ref["states"]
+    # run again: the states are found (J, T and E within 0.001) and not duplicated
+    run("XSHLJT", name + ".xml", "-n", str(ref["states_asked"]), cwd=str(tmp_path))
+    assert _states(str(tmp_path / (name + ".xml"))) == ref["states"]
+
+
+@pytest.mark.parametrize("name", LABEL_CASES)
+def test_dexhvjtcsb_labels_as_the_reference(gpu, tmp_path, name):
+    """DEXHVJTcsb (its own H and solve) on the same input: the reference XSHLJT's states in
+    the same order with the same J, P, T and names, each pointing at the state of the same
+    index in <name>.HH.EE; energies within 2e-4 MeV of its 6-digit values where the
+    reference's Davidson converged that pair (residual <= 1e-6), Ex where the lowest state
+    converged too.  (In the fp case the reference's coefficient test accepted pairs 2 and 7
+    with residuals 0.84 and 0.12; this build's strict solve converges them.)"""
+    from conftest import GOLDEN
+    ref = _label_case(name, tmp_path)
+    _, _, R, _ = oracle.read_ee(os.path.join(GOLDEN, name + "_EE.bin"))
+    k = str(ref["states_asked"])
+    run("DEXHVJTcsb", name + ".xml", "-n", k, "-b", k, cwd=str(tmp_path))
+    got = _states(str(tmp_path / (name + ".xml")))
+    assert len(got) == len(ref["states"])
+    ids = [int(dict(b["wavefunctions"][0])["id"]) for b in ref["states"]]
+    ground_ok = R[ids[0]] <= 1e-6
+    for a, b, i in zip(got, ref["states"], ids):
+        da, db = dict(a["attrs"]), dict(b["attrs"])
+        for key in ("J", "P", "T", "name"):
+            assert da[key] == db[key], (key, da, db)
+        if R[i] <= 1e-6:
+            assert abs(float(da["E"]) - float(db["E"])) < 2e-4, (da, db)
+            if ground_ok:
+                assert abs(float(da["Ex"]) - float(db["Ex"])) < 2e-4, (da, db)
+        assert [dict(w)["id"] for w in a["wavefunctions"]] == [dict(w)["id"] for w in b["wavefunctions"]]
+        assert dict(a["wavefunctions"][0])["file"] == name + ".HH.EE"

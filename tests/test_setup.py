@@ -156,3 +156,33 @@ def test_unsupported_hamiltonian_terms_are_refused(tmp_path, where, snippet):
     with pytest.raises(cosmo_amd.CosmoError) as e:
         cosmo_amd.System.from_xml(xml)
     assert e.value.code == cosmo_amd.EUNSUP
+
+
+def test_av_and_tz_give_the_valence_numbers(tmp_path):
+    """Without Nv / Zv the system's Av and Tz (physical (N - Z) / 2, as the reference writes
+    it) fix the valence numbers (include/xsm/xManyBodyStates.cxx:47-55): the same basis as
+    with Nv and Zv; a Tz of the wrong parity for Av is refused."""
+    import re
+    from cosmo_amd import synth
+    for (nv, zv, tz) in ((1, 3, "-1"), (2, 3, "-1/2"), (4, 2, "1")):
+        xml = synth.write_system(str(tmp_path), "sd", "s%d%d" % (nv, zv), nv, zv, (nv + zv) % 2, "+", 3)
+        n_ref = cosmo_amd.System.from_xml(xml).dim
+        txt = open(xml).read()
+        assert ('Tz="%s"' % tz) in txt
+        open(xml, "w").write(re.sub(r' Nv="\d+" Zv="\d+"', "", txt))
+        assert cosmo_amd.System.from_xml(xml).dim == n_ref
+    open(xml, "w").write(open(xml).read().replace('Tz="1"', 'Tz="1/2"'))
+    with pytest.raises(cosmo_amd.CosmoError):
+        cosmo_amd.System.from_xml(xml)
+
+
+def test_twice_spin_from_jmjp():
+    """2J from <J-J+> = J(J+1) - M(M+1) for integer and half-integer spins; -1 when J + M is
+    not an integer to within 0.1 (include/sm/QN.cxx:40-55)."""
+    lib = cosmo_amd.lib()
+    for J2 in range(0, 13):
+        for M2 in range(-J2, J2 + 1, 2):
+            x = (J2 * (J2 + 2) - M2 * (M2 + 2)) / 4.0
+            assert lib.cosmo_twice_spin(x + 0.03, M2) == J2
+    assert lib.cosmo_twice_spin(1.0, 0) == -1   # J = 0.618
+    assert lib.cosmo_twice_spin(-5.0, 0) == -1  # no real spin
