@@ -31,6 +31,10 @@ struct DevTables {
   const int32_t *pcfg_rank;   // [C(np,zv)] by colex rank
   const int32_t *pcfg_sec;    // [C(np,zv)] sector of each proton config
   const int32_t *ncfg_index;  // [C(nn,nv)] by colex rank (-1: not a basis block)
+  // truncated basis (null otherwise): block b keeps the proton configurations of sector
+  // ranks krank[kstart[b] .. kstart[b+1]), ascending
+  const int32_t *krank;
+  const int64_t *kstart;      // [nnconf+1]
   const double *vdense;       // [P*P]  V(annihilate pair pa, create pair pc)
   const uint32_t *vexist;     // [P*P/32 + 1] key present bits
   const int32_t *fstart;      // [P+1]  pure two-body terms by annihilated pair

@@ -684,7 +684,7 @@ static inline uint32_t below32(int i) { return (1u << i) - 1u; }
 // between species, or sectors or dense class blocks beyond what build_factored takes).
 int64_t factored_footprint(const System &s) {
   const int nn = s.nn, np = s.np, nblk = (int)s.nmask.size();
-  if (s.nv < 1 || s.zv < 1 || nn > 31 || np > 31) return -1;
+  if (s.nv < 1 || s.zv < 1 || nn > 31 || np > 31 || s.truncated()) return -1;
   for (size_t t = 0; t < s.esp_v.size(); ++t)
     if ((s.esp_k[2 * t] < nn) != (s.esp_k[2 * t + 1] < nn)) return -1;
   for (size_t t = 0; t < s.vspx_v.size(); ++t) {
@@ -719,6 +719,8 @@ int build_factored(Op &op, const System &s) {
   const int nn = s.nn, np = s.np;
   const int nblk = (int)s.nmask.size();
   if (s.nv < 1 || s.zv < 1) return fail(COSMO_EUNSUP, "factorised operator needs valence neutrons and protons");
+  // the product form needs every block to be its whole proton sector
+  if (s.truncated()) return fail(COSMO_EUNSUP, "factorised operator: truncated bases use the stored or row form");
   if (nn > 31 || np > 31) return fail(COSMO_EUNSUP, "factorised operator: more than 31 orbitals of a species");
   // every term must keep the neutron and proton counts
   for (size_t t = 0; t < s.esp_v.size(); ++t)

@@ -157,7 +157,17 @@ __device__ __forceinline__ int64_t ket_index(const DevTables &T, uint64_t K) {
   if (ni < 0) return -1;
   uint32_t pc = colex32(pm);
   if (T.pcfg_sec[pc] != T.nsector[ni]) return -1;
-  return T.noff[ni] + T.pcfg_rank[pc];
+  const int32_t r = T.pcfg_rank[pc];
+  if (!T.krank) return T.noff[ni] + r;
+  // truncated basis: the position of rank r among the block's kept ranks, if kept
+  int64_t lo = T.kstart[ni], hi = T.kstart[ni + 1];
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (T.krank[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo == T.kstart[ni + 1] || T.krank[lo] != r) return -1;
+  return T.noff[ni] + (lo - T.kstart[ni]);
 }
 
 __device__ __forceinline__ uint64_t bra_state(const DevTables &T, int64_t row) {
@@ -167,7 +177,8 @@ __device__ __forceinline__ uint64_t bra_state(const DevTables &T, int64_t row) {
     if (T.noff[mid] <= row) lo = mid; else hi = mid - 1;
   }
   int sec = T.nsector[lo];
-  uint32_t pm = T.pmask[T.psec_start[sec] + (row - T.noff[lo])];
+  const int64_t r = T.krank ? T.krank[T.kstart[lo] + (row - T.noff[lo])] : row - T.noff[lo];
+  uint32_t pm = T.pmask[T.psec_start[sec] + r];
   return (uint64_t)T.nmask[lo] | ((uint64_t)pm << T.nn);
 }
 
@@ -756,7 +767,8 @@ int upload_tables(const System &s, DevTablesOwner &o, hipStream_t st) {
          o_rt = put(rterm.data(), rterm.size() * 4), o_ohs = put(ohs.data(), ohs.size() * 4),
          o_oh = put(oh.data(), oh.size()), o_ols = put(ols.data(), ols.size() * 4), o_ol = put(ol.data(), ol.size()),
          o_esp = put(esp.data(), esp.size() * 8), o_espx = put(espx.data(), espx.size()),
-         o_pa = put(pair_a.data(), pair_a.size()), o_pb = put(pair_b.data(), pair_b.size());
+         o_pa = put(pair_a.data(), pair_a.size()), o_pb = put(pair_b.data(), pair_b.size()),
+         o_kr = put(s.krank.data(), s.krank.size() * 4), o_ks = put(s.kstart.data(), s.kstart.size() * 8);
   std::vector<uint8_t> host(total, 0);
   for (auto &p : pcs)
     if (p.bytes) memcpy(host.data() + p.off, p.src, p.bytes);
@@ -781,6 +793,8 @@ int upload_tables(const System &s, DevTablesOwner &o, hipStream_t st) {
   T.pcfg_rank = (const int32_t *)at(o_prank);
   T.pcfg_sec = (const int32_t *)at(o_psecof);
   T.ncfg_index = (const int32_t *)at(o_nidx);
+  T.krank = s.truncated() ? (const int32_t *)at(o_kr) : nullptr;
+  T.kstart = s.truncated() ? (const int64_t *)at(o_ks) : nullptr;
   T.vdense = (const double *)at(o_vd);
   T.vexist = (const uint32_t *)at(o_vx);
   T.fstart = (const int32_t *)at(o_fs);

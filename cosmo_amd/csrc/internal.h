@@ -53,6 +53,24 @@ struct System {
   std::vector<int32_t> pcfg_rank;  // colex(pmask) -> rank within its sector (-1 none)
   std::vector<int32_t> ncfg_index; // colex(nmask) -> index into nmask (-1 none)
   int64_t dim = 0;
+  // basis truncation of XsysmbsOMP (xmlshell/xmbs/XsysmbsOMP.cpp:212-303): oscillator
+  // quanta and occupation rejections.  When any is set, a block holds only its kept proton
+  // configurations: krank[kstart[b] .. kstart[b+1]) are their ranks within the sector,
+  // ascending (kstart empty: the full M-scheme basis)
+  struct Rejection {
+    std::vector<int> nlev, plev;  // neutron / proton levels summed (orbital index - 1)
+    int nmin = 0, nmax = 0;
+  };
+  bool hw_cut = false;
+  int hwmin = 0, hwmax = 0;
+  std::vector<Rejection> rejections;
+  std::vector<int32_t> krank;
+  std::vector<int64_t> kstart;
+  bool truncated() const { return !kstart.empty(); }
+  // sector rank of the proton configuration of basis row `row` in block b
+  int64_t prow(int b, int64_t row) const {
+    return truncated() ? krank[kstart[b] + (row - noff[b])] : row - noff[b];
+  }
   void build_basis();
 };
 

@@ -36,7 +36,16 @@ struct BraTables {
   const uint32_t *nmask, *pmask;
   const int64_t *noff, *psec_start;
   const int32_t *nsector;
+  const int32_t *krank;  // truncated source basis: kept sector ranks per block (else null)
+  const int64_t *kstart;
 };
+
+// |J+ z|^2 and |T+ z|^2 run over every state the raising operator reaches, as the
+// reference's PPMatrix::Sqr does, so the target basis is the untruncated one
+static void untruncate(System &t) {
+  t.hw_cut = false;
+  t.rejections.clear();
+}
 
 // partner[g] = orbital reached by the raising operator from g (-1 none); coef[g] its factor
 __global__ void k_raise(BraTables S, KetTables T, const uint32_t *binom, int64_t n, const int16_t *partner,
@@ -49,7 +58,8 @@ __global__ void k_raise(BraTables S, KetTables T, const uint32_t *binom, int64_t
     int mid = (lo + hi + 1) >> 1;
     if (S.noff[mid] <= row) lo = mid; else hi = mid - 1;
   }
-  uint32_t pm = S.pmask[S.psec_start[S.nsector[lo]] + (row - S.noff[lo])];
+  const int64_t pr = S.krank ? S.krank[S.kstart[lo] + (row - S.noff[lo])] : row - S.noff[lo];
+  uint32_t pm = S.pmask[S.psec_start[S.nsector[lo]] + pr];
   uint64_t B = (uint64_t)S.nmask[lo] | ((uint64_t)pm << S.nn);
   for (int g = 0; g < norb; ++g) {
     if (!((B >> g) & 1ull)) continue;
@@ -112,6 +122,10 @@ static int raise_norms(const System &s, const System &t, const std::vector<int16
   rc |= up(owned, s.noff, &S.noff);
   rc |= up(owned, s.psec_start, &S.psec_start);
   rc |= up(owned, s.nsector, &S.nsector);
+  if (s.truncated()) {
+    rc |= up(owned, s.krank, &S.krank);
+    rc |= up(owned, s.kstart, &S.kstart);
+  }
   T.nn = t.nn;
   T.nv = t.nv;
   T.zv = t.zv;
@@ -156,6 +170,7 @@ extern "C" int cosmo_system_labels(cosmo_system s, const double *evecs, int coun
   if (jm_jp) {
     System t = *s;
     t.twoM = s->twoM + 2;
+    untruncate(t);
     t.build_basis();
     std::vector<int16_t> partner(norb, -1);
     std::vector<double> coef(norb, 0.0);
@@ -172,6 +187,7 @@ extern "C" int cosmo_system_labels(cosmo_system s, const double *evecs, int coun
   }
   if (tm_tp) {
     System t = *s;
+    untruncate(t);
     t.nv = s->nv + 1;
     t.zv = s->zv - 1;
     if (t.zv < 0) {

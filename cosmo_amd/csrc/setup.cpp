@@ -583,17 +583,82 @@ void System::build_basis() {
   noff.assign(1, 0);
   nsector.clear();
   ncfg_index.assign(binom(nn, nv), -1);
+  krank.clear();
+  kstart.clear();
+  const bool cut = hw_cut || !rejections.empty();
+  // a configuration's signature: its oscillator quanta, then its occupation of each
+  // rejection's levels (the constraints only see these sums)
+  const int nr = (int)rejections.size();
+  auto signature = [&](uint32_t mask, int base, bool neutron) {
+    std::vector<int> g(nr + 1, 0);
+    for (uint32_t x = mask; x; x &= x - 1) {
+      const Orbital &o = orb[base + __builtin_ctz(x)];
+      g[0] += 2 * o.n + o.l;
+      for (int r = 0; r < nr; ++r) {
+        const std::vector<int> &lv = neutron ? rejections[r].nlev : rejections[r].plev;
+        if (std::find(lv.begin(), lv.end(), o.level) != lv.end()) ++g[r + 1];
+      }
+    }
+    return g;
+  };
+  // core quanta as CoreQuanta counts them (include/sm/SingleParticleStates.cxx:155-168):
+  // the first nv (zv) orbitals of the species in table order (its sort has an empty range)
+  int ncore = 0, pcore = 0;
+  for (int i = 0; i < nv && i < nn; ++i) ncore += 2 * orb[i].n + orb[i].l;
+  for (int i = 0; i < zv && i < np; ++i) pcore += 2 * orb[nn + i].n + orb[nn + i].l;
+  // per sector: the proton configurations grouped by signature, ranks ascending
+  std::vector<std::map<std::vector<int>, std::vector<int32_t>>> groups;
+  if (cut) {
+    groups.resize(psec_start.size() - 1);
+    for (size_t s = 0; s + 1 < psec_start.size(); ++s)
+      for (int64_t q = psec_start[s]; q < psec_start[s + 1]; ++q)
+        groups[s][signature(pmask[q], nn, false)].push_back((int32_t)(q - psec_start[s]));
+  }
+  std::vector<int32_t> keep;
   for (uint32_t m : ncomb) {
     int M, P;
     mpar(m, 0, M, P);
     auto it = secid.find({twoM - M, parity ^ P});
     if (it == secid.end()) continue;
     int s = it->second;
+    int64_t size = psec_start[s + 1] - psec_start[s];
+    if (cut) {
+      // the reference's partition filters: neutron and proton quanta each within hwmax of
+      // their core values (Partition::MakeQ), the total within [hwmin, hwmax] above both
+      // (QuantaMinMaxPurge), every rejection's occupation within [NMIN, NMAX]
+      // (TestRestriction, xmlshell/xmbs/PartitionClass.cxx:23-39)
+      const std::vector<int> gn = signature(m, 0, true);
+      keep.clear();
+      if (!hw_cut || (gn[0] >= ncore && gn[0] <= ncore + hwmax)) {
+        for (auto &g : groups[s]) {
+          const std::vector<int> &gp = g.first;
+          bool ok = true;
+          if (hw_cut) {
+            const int q = gn[0] + gp[0];
+            ok = gp[0] >= pcore && gp[0] <= pcore + hwmax && q >= ncore + pcore + hwmin &&
+                 q <= ncore + pcore + hwmax;
+          }
+          for (int r = 0; ok && r < nr; ++r) {
+            const int occ = gn[r + 1] + gp[r + 1];
+            ok = occ >= rejections[r].nmin && occ <= rejections[r].nmax;
+          }
+          if (ok) keep.insert(keep.end(), g.second.begin(), g.second.end());
+        }
+      }
+      if (keep.empty()) continue;
+      std::sort(keep.begin(), keep.end());
+      if (kstart.empty()) kstart.push_back(0);
+      krank.insert(krank.end(), keep.begin(), keep.end());
+      kstart.push_back((int64_t)krank.size());
+      size = (int64_t)keep.size();
+    }
     ncfg_index[colex_rank(m)] = (int32_t)nmask.size();
     nmask.push_back(m);
     nsector.push_back(s);
-    noff.push_back(noff.back() + (psec_start[s + 1] - psec_start[s]));
+    noff.push_back(noff.back() + size);
   }
+  // a truncation that keeps nothing still marks the basis as truncated
+  if (cut && kstart.empty()) kstart.push_back(0);
   dim = noff.back();
 }
 
@@ -616,6 +681,61 @@ static bool matrix_term(XNode *node, const char *name, double &x) {
   const double v = strtod(txt.c_str(), &end);
   if (end != txt.c_str()) x = v;
   return true;
+}
+
+// integer value read the same way (attribute, else child text)
+static bool node_int(XNode *node, const char *name, int &x) {
+  double v;
+  if (!matrix_term(node, name, v)) return false;
+  x = (int)v;
+  return true;
+}
+
+// the basis truncation of a <system> (xmlshell/xmbs/XsysmbsOMP.cpp:212-303): hwmax (an
+// attribute or element of the system, hwmin 0), or <hwrejection HWMAX HWMIN> (the first of
+// them); every <rejection NMIN NMAX> with its <orbital index type>: the levels of that XML
+// orbital index for both species ("pn"), protons ("p"), or neutrons ("n", and when the type
+// is absent).  An orbital index without a level of that species is refused.
+static int read_truncation(XNode *sysn, System &s) {
+  s.hw_cut = false;
+  s.rejections.clear();
+  if (node_int(sysn, "hwmax", s.hwmax)) {
+    s.hwmin = 0;
+    s.hw_cut = true;
+  }
+  if (XNode *h = sysn->child("hwrejection")) {
+    if (sysn->count("hwrejection") > 1) fprintf(stderr, "Only one rejection for hw is used \n");
+    if (!node_int(h, "HWMAX", s.hwmax) || !node_int(h, "HWMIN", s.hwmin))
+      return fail(COSMO_EIO, "<hwrejection> needs HWMAX and HWMIN");
+    s.hw_cut = true;
+  }
+  auto has_level = [&](int lev, bool neutron) {
+    for (int i = neutron ? 0 : s.nn; i < (neutron ? s.nn : s.nn + s.np); ++i)
+      if (s.orb[i].level == lev) return true;
+    return false;
+  };
+  for (auto &k : sysn->kids) {
+    if (k.name != "rejection") continue;
+    System::Rejection r;
+    if (!node_int(&k, "NMIN", r.nmin) || !node_int(&k, "NMAX", r.nmax))
+      return fail(COSMO_EIO, "<rejection> needs NMIN and NMAX");
+    for (auto &o : k.kids) {
+      if (o.name != "orbital") continue;
+      int ix;
+      if (!node_int(&o, "index", ix)) return fail(COSMO_EIO, "<rejection><orbital> without index");
+      const int lev = ix - 1;
+      const std::string type = o.get("type") ? o.get("type") : "n";
+      const bool n = type == "pn" || type == "n", p = type == "pn" || type == "p";
+      if (!n && !p) return fail(COSMO_EIO, "<rejection><orbital> type must be pn, p or n");
+      if ((n && !has_level(lev, true)) || (p && !has_level(lev, false)))
+        return fail(COSMO_EUNSUP, "<rejection> names orbital index " + std::to_string(ix) +
+                                      " for a species without that level");
+      if (n) r.nlev.push_back(lev);
+      if (p) r.plev.push_back(lev);
+    }
+    s.rejections.push_back(r);
+  }
+  return COSMO_OK;
 }
 
 static std::string dir_of(const std::string &p) {
@@ -854,6 +974,13 @@ int cosmo_system_from_xml(const char *xml_path, const char *sysname, cosmo_syste
     if (matrix_term(mat, t ? "TT" : "JJ", scale))
       add_raising_square(s->orb, t == 1, scale, s->esp_k, s->esp_v, s->vspx_k, s->vspx_v);
   }
+  {
+    const int rc = read_truncation(sysn, *s);
+    if (rc) {
+      delete s;
+      return rc;
+    }
+  }
   s->build_basis();
   *out = s;
   return COSMO_OK;
@@ -880,14 +1007,14 @@ int cosmo_system_info(cosmo_system s, int *nparticles, int *nsps, int *nn, int *
 int cosmo_system_basis(cosmo_system s, uint16_t *out) {
   if (!s || !out) return fail(COSMO_EINVAL, "null argument");
   const int A = s->nv + s->zv;
-  int64_t r = 0;
   for (size_t i = 0; i < s->nmask.size(); ++i) {
-    int sec = s->nsector[i];
-    for (int64_t q = s->psec_start[sec]; q < s->psec_start[sec + 1]; ++q, ++r) {
+    const int64_t q0 = s->psec_start[s->nsector[i]];
+    for (int64_t r = s->noff[i]; r < s->noff[i + 1]; ++r) {
       uint16_t *row = out + r * A;
       int c = 0;
       for (uint32_t m = s->nmask[i]; m; m &= m - 1) row[c++] = (uint16_t)__builtin_ctz(m);
-      for (uint32_t m = s->pmask[q]; m; m &= m - 1) row[c++] = (uint16_t)(s->nn + __builtin_ctz(m));
+      for (uint32_t m = s->pmask[q0 + s->prow((int)i, r)]; m; m &= m - 1)
+        row[c++] = (uint16_t)(s->nn + __builtin_ctz(m));
     }
   }
   return COSMO_OK;
@@ -949,10 +1076,11 @@ int cosmo_system_check_mbs(cosmo_system s, const char *path) {
   int extra = fgetc(f);
   fclose(f);
   if (got != z.size() || extra != EOF) return fail(COSMO_EIO, "Read:ManyBodyStates file-size mismatch");
-  if ((int64_t)n != s->dim) return fail(COSMO_EUNSUP, ".mbs basis is not the full M-scheme basis (truncated)");
+  const char *what = s->truncated() ? "the system's truncated basis" : "the full M-scheme basis";
+  if ((int64_t)n != s->dim) return fail(COSMO_EUNSUP, std::string(".mbs basis size differs from ") + what);
   std::vector<uint16_t> mine(z.size());
   cosmo_system_basis(s, mine.data());
-  if (mine != z) return fail(COSMO_EUNSUP, ".mbs basis differs from the full M-scheme basis in reference order");
+  if (mine != z) return fail(COSMO_EUNSUP, std::string(".mbs basis differs from ") + what + " in reference order");
   return COSMO_OK;
 }
 

@@ -25,10 +25,13 @@ import numpy as np
 SHELLS = {
     "sd": [("0d5", 0, 2, 5), ("1s1", 1, 0, 1), ("0d3", 0, 2, 3)],
     "fp": [("0f7", 0, 3, 7), ("1p3", 1, 1, 3), ("0f5", 0, 3, 5), ("1p1", 1, 1, 1)],
+    # two oscillator shells (0p: 1 quantum, sd: 2) for truncations by quanta
+    "psd": [("0p3", 0, 1, 3), ("0p1", 0, 1, 1), ("0d5", 0, 2, 5), ("1s1", 1, 0, 1), ("0d3", 0, 2, 3)],
 }
-CORES = {"sd": ("16O", 8, 8, 16.0, 18.0, 0.3), "fp": ("40Ca", 20, 20, 40.0, 42.0, 0.3)}
-SPE_RANGE = {"sd": (-4.0, 2.0), "fp": (-2.0, 6.0)}
-TBME_SIGMA = {"sd": 1.5, "fp": 1.2}
+CORES = {"sd": ("16O", 8, 8, 16.0, 18.0, 0.3), "fp": ("40Ca", 20, 20, 40.0, 42.0, 0.3),
+         "psd": ("4He", 2, 2, 4.0, 6.0, 0.3)}
+SPE_RANGE = {"sd": (-4.0, 2.0), "fp": (-2.0, 6.0), "psd": (-6.0, 4.0)}
+TBME_SIGMA = {"sd": 1.5, "fp": 1.2, "psd": 1.5}
 
 
 def allowed_tbmes(shell: str):
@@ -92,11 +95,13 @@ def _jstr(twoj: int) -> str:
 
 def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
                intfile: str, mbs: str | None = None, matrix: str | None = None,
-               matrix_attrs: dict | None = None, system_attrs: dict | None = None) -> str:
+               matrix_attrs: dict | None = None, system_attrs: dict | None = None,
+               system_children: str = "") -> str:
     """CoSMo system XML (layout of include/xsm/schema.h:6-62).  matrix_attrs adds
     attributes such as JJ / TT to the system's <matrix> node (written when either
     matrix or matrix_attrs is given); system_attrs adds attributes such as the J / T
-    filters of the state labelling to the <system> node."""
+    filters of the state labelling to the <system> node, system_children elements such as
+    the <hwrejection> / <rejection> basis truncations."""
     core, cn, cz, acore, _, _ = CORES[shell]
     av = nv + zv
     def spin_str(twice):  # a spin as the reference's XML writes it: "1", "3/2", "-1/2"
@@ -114,6 +119,8 @@ def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
     if matrix or matrix_attrs:
         attrs = "".join(' %s="%s"' % kv for kv in (matrix_attrs or {}).items())
         extra += '\t\t<matrix name="%s" format="pp"%s/>\n' % (matrix or name + ".HH", attrs)
+    if system_children:
+        extra += "".join("\t\t%s\n" % line for line in system_children.strip().splitlines())
     return (
         '<?xml version="1.0" encoding="ISO-8859-1"?>\n<cosmo>\n'
         '\t<valencespace name="%s">\n\t\t<core name="%s" N="%d" Z="%d"/>\n%s\n\t</valencespace>\n'
@@ -126,7 +133,7 @@ def system_xml(shell: str, name: str, nv: int, zv: int, twojz: int, parity: str,
 
 def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: int = 0,
                  parity: str = "+", seed: int = 0, matrix_attrs: dict | None = None,
-                 system_attrs: dict | None = None) -> str:
+                 system_attrs: dict | None = None, system_children: str = "") -> str:
     """Write ``<name>.int`` and ``<name>.xml`` into dirpath; return the XML path."""
     os.makedirs(dirpath, exist_ok=True)
     inter = make_interaction(shell, seed)
@@ -135,7 +142,7 @@ def write_system(dirpath: str, shell: str, name: str, nv: int, zv: int, twojz: i
     xml = os.path.join(dirpath, name + ".xml")
     with open(xml, "w") as f:
         f.write(system_xml(shell, name, nv, zv, twojz, parity, name + ".int", matrix_attrs=matrix_attrs,
-                           system_attrs=system_attrs))
+                           system_attrs=system_attrs, system_children=system_children))
     return xml
 
 

@@ -86,9 +86,11 @@ int cosmo_system_labels(cosmo_system s, const double *evecs, int count, double *
 /* 2J from <J-J+> and 2M: sqrt(4x + 2M(2M+2) + 1) - 1 rounded, or -1 when J + M is not an
  * integer to within 0.1 (include/sm/QN.cxx:40-55). */
 int cosmo_twice_spin(double x, int twoM);
-/* .mbs file I/O (include/sm/ManyBodyStates.cxx:199-246).  cosmo_system_check_mbs fails
- * with COSMO_EUNSUP when the file's basis differs from the full M-scheme basis
- * (truncated bases are not handled by the rank-table builder). */
+/* .mbs file I/O (include/sm/ManyBodyStates.cxx:199-246).  The basis is the system's: the
+ * full M-scheme basis, or the one its <system> truncates by oscillator quanta (hwmax,
+ * <hwrejection HWMAX HWMIN>) and occupation rejections (<rejection NMIN NMAX><orbital
+ * index type/>) as XsysmbsOMP does (xmlshell/xmbs/XsysmbsOMP.cpp:212-303), in the same order.
+ * cosmo_system_check_mbs fails with COSMO_EUNSUP when the file holds another basis. */
 int cosmo_system_write_mbs(cosmo_system s, const char *path);
 int cosmo_system_check_mbs(cosmo_system s, const char *path);
 

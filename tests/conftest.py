@@ -11,6 +11,18 @@ sys.path.insert(0, ROOT)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 GOLDEN_CASES = ["sd2p2n_s11", "sd3p1n_m1_s7", "fp2p2n_s5", "fp3p2n_m1_s3", "sd4p4n_s0"]
 FULL_CASES = ["sd2p2n_s11", "sd3p1n_m1_s7", "fp2p2n_s5"]
+# truncated bases (hw window, hw cut with parity -, an hw window above its minimum, the
+# hwmax attribute on odd A,
+# occupation rejections in sd and fp); fixtures from tests/golden/make_golden.py trunc
+TRUNC_CASES = ["psd2p2n_hw2", "psd2p2n_neg_hw1", "psd2p2n_hw22", "psd3p2n_m1_hwmax2", "sd3p3n_rej",
+               "fp3p2n_m1_rej"]
+
+
+def trunc_case(name):
+    """(xml path, meta, arrays) of a truncated-basis fixture."""
+    meta = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    arrays = dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+    return os.path.join(GOLDEN, name + ".xml"), meta, arrays
 
 
 def pytest_configure(config):

@@ -4,9 +4,9 @@ Run only where the CoSMo source tree is present at /root/reference (the committe
 fixtures are what the tests read elsewhere): ``make -C oracle ref`` builds XsysmbsOMP, XHH+JJ, davidson_vfile, XSHLJT and ref_harness
 into oracle/_ref from those sources, then this script runs them on seeded synthetic
 systems and stores inputs and outputs as small data files (npz / json / text).
-``python make_golden.py csb|hh|traj|matrix|labels`` only adds the CSB file keys, the .HH row
-product, the Davidson counter runs, the <matrix> JJ/TT cases or the state labelling
-(XSHLJT) cases.
+``python make_golden.py csb|hh|traj|matrix|labels|trunc`` only adds the CSB file keys, the
+.HH row product, the Davidson counter runs, the <matrix> JJ/TT cases, the state labelling
+(XSHLJT) cases or the truncated-basis cases.
 """
 import hashlib
 import json
@@ -279,8 +279,56 @@ def labels_only():
             shutil.rmtree(d)
 
 
+# truncated bases (XsysmbsOMP's hw and occupation rejections): name, shell, Nv, Zv, 2Jz,
+# parity, seed, <system> attributes, <system> children
+TRUNC_CASES = [
+    ("psd2p2n_hw2", "psd", 2, 2, 0, "+", 4, {}, '<hwrejection HWMAX="2" HWMIN="0"/>'),
+    ("psd2p2n_neg_hw1", "psd", 2, 2, 0, "-", 4, {}, '<hwrejection HWMAX="1" HWMIN="0"/>'),
+    ("psd2p2n_hw22", "psd", 2, 2, 0, "+", 4, {}, '<hwrejection HWMAX="2" HWMIN="2"/>'),
+    ("psd3p2n_m1_hwmax2", "psd", 2, 3, 1, "-", 9, {"hwmax": "2"}, ""),
+    ("sd3p3n_rej", "sd", 3, 3, 0, "+", 6, {},
+     '<rejection NMIN="0" NMAX="3">\n<orbital index="1" type="pn"/>\n</rejection>\n'
+     '<rejection NMIN="1" NMAX="4">\n<orbital index="2"/>\n<orbital index="3" type="p"/>\n</rejection>'),
+    ("fp3p2n_m1_rej", "fp", 2, 3, 1, "-", 3, {},
+     '<rejection NMIN="0" NMAX="1">\n<orbital index="3" type="pn"/>\n<orbital index="4" type="pn"/>\n</rejection>'),
+]
+
+
+def trunc_only():
+    """<name>.xml/.int, <name>.json (dimension and SHA-256 of the .mbs XsysmbsOMP writes,
+    size and SHA-256 of the .HH) and <name>.npz (the .HH rows XHH+JJ writes on that
+    truncated basis: all of them up to n = 4100, 64 sampled rows above)."""
+    for (name, shell, nv, zv, twojz, par, seed, sattrs, kids) in TRUNC_CASES:
+        d = tempfile.mkdtemp()
+        try:
+            xml = synth.write_system(d, shell, name, nv, zv, twojz, par, seed, system_attrs=sattrs,
+                                     system_children=kids)
+            shutil.copy(os.path.join(d, name + ".int"), os.path.join(OUT, name + ".int"))
+            shutil.copy(xml, os.path.join(OUT, name + ".xml"))
+            run([os.path.join(REF, "XsysmbsOMP"), name + ".xml"], d)
+            run([os.path.join(REF, "XHH+JJ"), name + ".xml"], d)
+            rp, cols, vals = oracle.read_hh(os.path.join(d, name + ".HH"))
+            mbs = oracle.read_mbs(os.path.join(d, name + ".mbs"))
+            if len(mbs) <= 4100:
+                np.savez_compressed(os.path.join(OUT, name + ".npz"), hh_rowptr=rp, hh_cols=cols, hh_vals=vals)
+            else:  # 64 sampled rows
+                rows = np.sort(np.random.default_rng(2).choice(len(mbs), 64, replace=False))
+                np.savez_compressed(os.path.join(OUT, name + ".npz"), sample_rows=rows,
+                                    sample_rowptr=np.concatenate([[0], np.cumsum(rp[rows + 1] - rp[rows])]),
+                                    sample_cols=np.concatenate([cols[rp[r]:rp[r + 1]] for r in rows]),
+                                    sample_vals=np.concatenate([vals[rp[r]:rp[r + 1]] for r in rows]))
+            with open(os.path.join(OUT, name + ".json"), "w") as f:
+                json.dump(dict(name=name, shell=shell, nv=nv, zv=zv, twojz=twojz, parity=par, seed=seed,
+                               system_attrs=sattrs, n=int(len(mbs)), mbs_sha256=sha(os.path.join(d, name + ".mbs")),
+                               hh_nnz=int(rp[-1]), hh_sha256=sha(os.path.join(d, name + ".HH"))), f, indent=1)
+            print(name, len(mbs), rp[-1])
+        finally:
+            shutil.rmtree(d)
+
+
 if __name__ == "__main__":
-    modes = {"csb": csb_only, "hh": hh_only, "traj": traj_only, "matrix": matrix_only, "labels": labels_only}
+    modes = {"csb": csb_only, "hh": hh_only, "traj": traj_only, "matrix": matrix_only, "labels": labels_only,
+             "trunc": trunc_only}
     if sys.argv[1:2] and sys.argv[1] in modes:
         modes[sys.argv[1]]()
     else:

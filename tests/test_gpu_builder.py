@@ -114,3 +114,64 @@ def test_jj_tt_rows_match_reference_hh(gpu, golden, name):
     for flags in (gpu.OP_STORED, gpu.OP_FACTORED):
         Y = gpu.Operator.build(s, flags).apply(X)
         assert np.max(np.abs(Y - Yr)) <= 1e-12 * np.max(np.abs(Yr))
+
+
+@pytest.mark.parametrize("name", __import__("conftest").TRUNC_CASES)
+def test_truncated_rows(gpu, name):
+    """Truncated bases (hw and occupation rejections): the GPU rows bit-exact against the
+    oracle's on the same basis (with and without the 1e-14 drop), and against the rows
+    the reference's XHH+JJ writes (structure exact, values within 1e-12); the row form
+    regenerates the same products; the product form is refused."""
+    from conftest import trunc_case
+    xml, meta, a = trunc_case(name)
+    s = gpu.System.from_xml(xml)
+    basis = s.basis()
+    o = oracle.System(xml)
+    n = s.dim
+    for keep in (False, True):
+        op = gpu.Operator.build(s, gpu.OP_KEEP_SMALL if keep else gpu.OP_STORED)
+        rp, c, v = op.upper_rows()
+        orp, oc, ov = o.h_upper(basis=basis, drop_small=not keep)
+        assert np.array_equal(rp, orp) and np.array_equal(c, oc)
+        assert np.array_equal(v.view(np.uint64), ov.view(np.uint64))
+        if keep and "hh_rowptr" in a:
+            assert np.array_equal(rp, a["hh_rowptr"]) and np.array_equal(c, a["hh_cols"])
+            assert np.max(np.abs(v - a["hh_vals"])) < 1e-12
+        if keep and "sample_rows" in a:
+            for i, r in enumerate(a["sample_rows"]):
+                s0, s1 = a["sample_rowptr"][i], a["sample_rowptr"][i + 1]
+                assert np.array_equal(c[rp[r]:rp[r + 1]], a["sample_cols"][s0:s1])
+                assert np.max(np.abs(v[rp[r]:rp[r + 1]] - a["sample_vals"][s0:s1])) < 1e-12
+        if not keep:
+            X = np.random.default_rng(1).standard_normal((n, 5))
+            ref = oracle.csb_times(n, orp, oc, ov, X)
+            y = op.apply(X, gpu.PREC_FP64)
+            assert np.max(np.abs(y - ref)) <= 1e-12 * np.abs(ref).max()
+            vop = gpu.Operator.build(s, gpu.OP_VIRTUAL)
+            assert np.max(np.abs(vop.apply(X, gpu.PREC_FP64) - y)) <= 1e-12 * np.abs(ref).max()
+            vop.close()
+        op.close()
+    with pytest.raises(gpu.CosmoError):
+        gpu.Operator.build(s, gpu.OP_FACTORED)
+
+
+@pytest.mark.parametrize("name", ["psd2p2n_hw2", "sd3p3n_rej"])
+def test_truncated_solve_and_labels(gpu, name):
+    """Davidson on a truncated basis: the lowest eigenvalues of the oracle's dense H on the
+    same basis within 1e-8, residuals <= 1e-8; J labels integral (hw and occupation
+    rejections keep every level's occupation, so J is a good quantum number there)."""
+    from conftest import trunc_case
+    xml, meta, a = trunc_case(name)
+    s = gpu.System.from_xml(xml)
+    o = oracle.System(xml)
+    rp, c, v = o.h_upper(basis=s.basis(), drop_small=True)
+    w = np.linalg.eigvalsh(oracle.dense_from_upper(s.dim, rp, c, v))
+    op = gpu.Operator.build(s)
+    ev, de, rs, vec, st = gpu.davidson(op, 6, lim=40)
+    assert np.max(np.abs(ev - w[:6])) < 1e-8 and rs.max() <= 1e-8
+    jj, tt = s.labels(vec)
+    lib = gpu.lib()
+    for x in jj:
+        J2 = lib.cosmo_twice_spin(float(x), 0)
+        assert J2 >= 0 and abs(x - (J2 * (J2 + 2)) / 4.0) < 1e-6  # M = 0
+    op.close()

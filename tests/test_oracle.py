@@ -314,3 +314,25 @@ def test_oracle_ee_reader_round_trip(tmp_path):
     open(p, "r+b").truncate(4 + 32 + 8 * n + 16)
     with pytest.raises(Exception):
         oracle.read_ee(p)
+
+
+@pytest.mark.parametrize("name", __import__("conftest").TRUNC_CASES)
+def test_oracle_rows_on_truncated_basis_vs_reference_hh(name):
+    """The oracle's row generator on a truncated basis (the system's, whose .mbs is the
+    reference's byte for byte) gives the rows the reference's XHH+JJ writes for it: hops
+    leaving the basis dropped, structure exact, values within 1e-12."""
+    import cosmo_amd
+    from conftest import trunc_case
+    xml, meta, a = trunc_case(name)
+    basis = cosmo_amd.System.from_xml(xml).basis()
+    o = oracle.System(xml)
+    if "hh_rowptr" in a:
+        rp, c, v = o.h_upper(basis=basis, drop_small=False)
+        assert np.array_equal(rp, a["hh_rowptr"]) and np.array_equal(c, a["hh_cols"])
+        assert np.max(np.abs(v - a["hh_vals"])) < 1e-12
+    else:
+        for i, r in enumerate(a["sample_rows"]):
+            rp, c, v = o.h_upper(basis=basis, drop_small=False, r0=int(r), r1=int(r) + 1)
+            s0, s1 = a["sample_rowptr"][i], a["sample_rowptr"][i + 1]
+            assert np.array_equal(c, a["sample_cols"][s0:s1])
+            assert np.max(np.abs(v - a["sample_vals"][s0:s1])) < 1e-12

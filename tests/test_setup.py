@@ -186,3 +186,43 @@ def test_twice_spin_from_jmjp():
             assert lib.cosmo_twice_spin(x + 0.03, M2) == J2
     assert lib.cosmo_twice_spin(1.0, 0) == -1   # J = 0.618
     assert lib.cosmo_twice_spin(-5.0, 0) == -1  # no real spin
+
+
+@pytest.mark.parametrize("name", __import__("conftest").TRUNC_CASES)
+def test_truncated_basis_matches_reference_mbs(name, tmp_path):
+    """The hw and occupation rejections of XsysmbsOMP (xmlshell/xmbs/XsysmbsOMP.cpp:212-303):
+    the basis the system builds is the reference's .mbs byte for byte; that file passes the
+    check, and the same system without its truncation refuses it."""
+    import re
+    from conftest import trunc_case
+    xml, meta, _ = trunc_case(name)
+    s = cosmo_amd.System.from_xml(xml)
+    assert s.dim == meta["n"]
+    p = str(tmp_path / "t.mbs")
+    s.write_mbs(p)
+    assert hashlib.sha256(open(p, "rb").read()).hexdigest() == meta["mbs_sha256"]
+    s.check_mbs(p)
+    txt = open(xml).read()
+    full = re.sub(r'\s*<hwrejection[^>]*/>|\s*<rejection.*?</rejection>', "", txt, flags=re.S)
+    full = full.replace(' hwmax="2"', "")
+    q = str(tmp_path / "full.xml")
+    open(q, "w").write(full)
+    import shutil
+    shutil.copy(xml[:-4] + ".int", str(tmp_path / (name + ".int")))
+    f = cosmo_amd.System.from_xml(q)
+    assert f.dim > s.dim
+    with pytest.raises(cosmo_amd.CosmoError):
+        f.check_mbs(p)
+
+
+def test_truncation_input_errors(tmp_path):
+    """Malformed truncations are refused: a rejection without NMAX, an orbital index of a
+    level the species does not have, an hwrejection without HWMIN."""
+    from cosmo_amd import synth
+    bad = ['<rejection NMIN="0">\n<orbital index="1"/>\n</rejection>',
+           '<rejection NMIN="0" NMAX="2">\n<orbital index="9" type="pn"/>\n</rejection>',
+           '<hwrejection HWMAX="2"/>']
+    for i, kids in enumerate(bad):
+        xml = synth.write_system(str(tmp_path), "psd", "b%d" % i, 2, 2, 0, "+", 1, system_children=kids)
+        with pytest.raises(cosmo_amd.CosmoError):
+            cosmo_amd.System.from_xml(xml)
