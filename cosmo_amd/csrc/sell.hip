@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -50,6 +51,7 @@ struct Sell {
   float *v32 = nullptr;          // [E]
   double *v64 = nullptr;         // [E]
   size_t bytes = 0;
+  std::map<int, int> chunk;      // (element size, pitch) -> XCD chunk chosen by timing
 };
 
 struct SellView {
@@ -564,12 +566,12 @@ int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
   return (int64_t)(1.7 * (double)nnz) * (2 + 4 + 8) + 64 * n;
 }
 
-int g_sell_chunk = 8;  // XCD chunk of row blocks (COSMO_SELL_CHUNK)
+int g_sell_chunk = 0;  // XCD chunk of row blocks (COSMO_SELL_CHUNK; 0: chosen by timing)
 int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
 int g_sell_stats = 0;  // COSMO_SELL_STATS=1: the layout's sizes on stderr when it is built
 
 template <typename T, int LD>
-static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {
+static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, int chunk, hipStream_t st) {
   SellView v{s.nblk, s.maxrows, s.maxtiles, s.nwaves * 64, s.boff, s.nbr_start, s.nbr_tile, s.qstart, s.sl_base,
              s.cols};
   const size_t lds = 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(s);
@@ -580,9 +582,56 @@ static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, hi
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_done = true;
   }
-  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, g_sell_chunk);
+  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk);
   HIPCHK(hipGetLastError());
   return COSMO_OK;
+}
+
+// One product with the XCD chunk of this (precision, pitch): on the first call every chunk
+// of SELL_CHUNKS is timed (twice each) and the fastest kept.  The chunk only changes which
+// XCD runs which row block, never a row's summation, so every launch writes the same Y.
+static constexpr int SELL_CHUNKS[] = {1, 2, 4, 8, 16, 32};
+template <typename T, int LD>
+static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {
+  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, X, Y, m, g_sell_chunk, st);
+  const int key = (int)sizeof(T) * 4096 + LD;
+  int chunk;
+  {
+    std::lock_guard<std::mutex> lk(g_sell_mu);
+    auto it = s.chunk.find(key);
+    if (it != s.chunk.end()) {
+      chunk = it->second;
+    } else {
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      float best_ms = 1e30f;
+      chunk = 8;
+      for (int c : SELL_CHUNKS)
+        for (int rep = 0; rep < 2; ++rep) {
+          HIPCHK(hipEventRecord(e0, st));
+          const int rc = launch_sell<T, LD>(s, vals, X, Y, m, c, st);
+          if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+          }
+          HIPCHK(hipEventRecord(e1, st));
+          HIPCHK(hipEventSynchronize(e1));
+          float t = 0;
+          HIPCHK(hipEventElapsedTime(&t, e0, e1));
+          if (t < best_ms) best_ms = t, chunk = c;
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      s.chunk.emplace(key, chunk);
+      if (g_sell_stats)
+        fprintf(stderr, "[sell] XCD chunk %d for %d-byte values at pitch %d (%.3f ms)\n", chunk, (int)sizeof(T), LD,
+                best_ms);
+      return COSMO_OK;  // the timed launches wrote Y
+    }
+  }
+  return launch_sell<T, LD>(s, vals, X, Y, m, chunk, st);
 }
 
 // Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
@@ -604,20 +653,20 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
     const int rc = sell_values(op, f32, st);
     if (rc) return rc;
   }
-  const Sell &s = *(const Sell *)op.sell;
+  Sell &s = *(Sell *)op.sell;
   if (f32) {
     const float *x = (const float *)X;
     float *y = (float *)Y;
-    if (ld == 4) return launch_sell<float, 4>(s, s.v32, x, y, m, st);
-    if (ld == 8) return launch_sell<float, 8>(s, s.v32, x, y, m, st);
-    return launch_sell<float, 16>(s, s.v32, x, y, m, st);
+    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, x, y, m, st);
+    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, x, y, m, st);
+    return launch_sell_tuned<float, 16>(s, s.v32, x, y, m, st);
   }
   const double *x = (const double *)X;
   double *y = (double *)Y;
-  if (ld == 2) return launch_sell<double, 2>(s, s.v64, x, y, m, st);
-  if (ld == 4) return launch_sell<double, 4>(s, s.v64, x, y, m, st);
-  if (ld == 8) return launch_sell<double, 8>(s, s.v64, x, y, m, st);
-  return launch_sell<double, 16>(s, s.v64, x, y, m, st);
+  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, x, y, m, st);
+  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, x, y, m, st);
+  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, x, y, m, st);
+  return launch_sell_tuned<double, 16>(s, s.v64, x, y, m, st);
 }
 
 }  // namespace cosmo
