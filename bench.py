@@ -83,8 +83,6 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline row sample (0: n/8)")
     ap.add_argument("--layout", choices=["auto", "rows", "tiled"], default="auto",
                     help="stored-H kernel: auto (tiled from n = 65536), rows (one wave per row) or tiled")
-    ap.add_argument("--slabs", type=int, default=-1,
-                    help="column slabs of the tiled product (-1: library default, 0: none)")
     ap.add_argument("--traffic-json", default=None,
                     help="HBM bytes per launch of the stored kernel from rocprofv3 counters "
                          "(default: the committed summary for this kernel and workload, if any)")
@@ -140,8 +138,6 @@ def main():
     op = ca.Operator.build(sysm)
     if args.layout != "auto":
         op.set_layout({"rows": 1, "tiled": 2}[args.layout])
-    if args.slabs >= 0:
-        op.set_slabs(args.slabs)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     nnz_u, nnz_f = op.nnz()

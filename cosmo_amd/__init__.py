@@ -104,7 +104,6 @@ SIGNATURES = {
     "cosmo_op_write_csb": (_I, [_P, _S, _I]),
     "cosmo_op_timings": (_I, [_P, _P, _P]),
     "cosmo_op_set_layout": (_I, [_P, _I]),
-    "cosmo_op_set_slabs": (_I, [_P, _I]),
     "cosmo_op_dense_eigen": (_I, [_P, _P, _P]),
     "cosmo_dav_defaults": (_I, [_P, _I]),
     "cosmo_davidson": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
@@ -349,10 +348,6 @@ class Operator:
     def set_layout(self, layout):
         """Stored product kernel: 0 automatic, 1 rows, 2 tiled."""
         check(lib().cosmo_op_set_layout(self.h, int(layout)))
-
-    def set_slabs(self, slabs):
-        """Column slabs of the tiled product: -1 library default, 0 none, else the count."""
-        check(lib().cosmo_op_set_slabs(self.h, int(slabs)))
 
     def timings(self):
         b, a = ctypes.c_double(), ctypes.c_double()

@@ -33,7 +33,6 @@ struct Op {
   hipStream_t home = nullptr;  // library stream of the creating thread
   bool foreign = false;        // used on another stream: destroy synchronises the device
   int layout = 0;              // stored product: 0 automatic, 1 rows, 2 tiled (sell.hip)
-  int slabs = -1;              // tiled product's column slabs: -1 the library default, 0 none
   double build_ms = 0, last_apply_ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -56,7 +55,7 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
 void free_sell(Op &op);
 size_t sell_bytes(const Op &op);
 int64_t sell_footprint_bound(int64_t n, int64_t nnz);
-extern int g_sell_chunk, g_sell_off, g_sell_stats, g_sell_slabs;
+extern int g_sell_chunk, g_sell_off, g_sell_stats;
 int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
 int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);

@@ -702,11 +702,6 @@ int cosmo_op_set_layout(cosmo_op op, int layout) {
   return COSMO_OK;
 }
 
-int cosmo_op_set_slabs(cosmo_op op, int slabs) {
-  if (!op || slabs < -1 || slabs > 4096) return fail(COSMO_EINVAL, "cosmo_op_set_slabs: bad args");
-  op->slabs = slabs;
-  return COSMO_OK;
-}
 
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms) {
   if (!op) return fail(COSMO_EINVAL, "null operator");

@@ -39,19 +39,6 @@ namespace cosmo {
 #define SELL_PF 4         // entry slots per lane loaded one tile ahead
 #define SELL_MIN_N 65536  // smaller operators keep the row kernel
 
-// Column slabs: the column tiles cut into S ranges of equal weight (the bytes their pairs
-// move), slab s owned by XCD s % 8.  A workgroup item is one row block's tiles inside one
-// slab; each XCD runs its slabs' items slab after slab, so the X tiles of the slab in work
-// stay in that XCD's L2 instead of coming from the Infinity Cache for every pair.  A row
-// block whose tiles span several slabs writes one set of partial rows per slab, summed in
-// slab order by a second kernel (deterministic: the same bits on every call).
-struct SellSlabs {
-  int S = 0, nitems = 0, grid = 0, nred = 0;
-  int64_t prows = 0;         // partial rows of all split row blocks
-  int4 *items = nullptr;     // [2 grid] the item of workgroup i (XCD i % 8; tiles -1 for none)
-  int4 *red = nullptr;       // [nred] (row block, first partial row, slabs, 0)
-};
-
 struct Sell {
   int nblk = 0, maxrows = 0, nwaves = 0, maxtiles = 0;
   int64_t K = 0, Q = 0, E = 0, Ealloc = 0;
@@ -65,21 +52,15 @@ struct Sell {
   double *v64 = nullptr;         // [E]
   uint32_t *pair_c0 = nullptr;   // [K] first X row of each pair's column tile
   int32_t *pair_rows = nullptr;  // [K] its rows
-  int4 *items = nullptr;         // [2 nblk] one item per whole row block
+  int4 *items = nullptr;         // [2 nblk] the item of each row block
   size_t bytes = 0;
   std::map<int, int> chunk;      // (element size, pitch) -> XCD chunk chosen by timing
-  // host copies of the block structure for the slab plans: per (row block, tile) pair its
-  // slots in units of 64
-  std::vector<int64_t> h_boff, h_qstart;
-  std::vector<int32_t> h_nbr_start, h_nbr_tile, h_pair_slots;
-  std::map<int, SellSlabs> slabs;                           // by slab count
-  std::map<hipStream_t, std::pair<void *, size_t>> part;    // partial rows, one buffer per stream
 };
 
-// An item is two int4: (first pair, tiles, rows, first partial row or -1) and (first row,
-// first slice, 0, 0); pairs are the (row block, column tile) pairs in block-major order.
+// A row block's item is two int4: (first pair, tiles, rows, 0) and (first row, first
+// slice, 0, 0); pairs are the (row block, column tile) pairs in block-major order.
 struct SellView {
-  int maxtiles, nthr, by_block;  // by_block: the item of workgroup i is row block sell_block(chunk)
+  int maxtiles, nthr;
   const int4 *items;
   const uint32_t *pair_c0;       // per pair: the tile's first X row
   const int32_t *pair_rows;      // and its rows
@@ -222,25 +203,22 @@ __device__ __forceinline__ int sell_block(int chunk) {
 template <typename T, int LD>
 __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals,
                                                             const T *__restrict__ X, T *__restrict__ Y, int m,
-                                                            int chunk, T *__restrict__ P) {
+                                                            int chunk) {
   typedef typename V16t<T>::type V;
   constexpr int VPR = (int)(LD * sizeof(T) / 16);  // 16-byte pieces per X row
   constexpr int VW = (int)(16 / sizeof(T));        // elements per piece
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  // the work item (one record, the only load before the tile table): a whole row block,
-  // or with column slabs a run of a row block's tiles, written to partial rows from prow
-  // when the block is split; K < 0 marks a workgroup without an item
-  const int it = A.by_block ? sell_block(chunk) : (int)blockIdx.x;
-  const int4 d0 = A.items[2 * it], d1 = A.items[2 * it + 1];
+  // the row block's record: the only load before the tile table
+  const int b = sell_block(chunk);
+  const int4 d0 = A.items[2 * b], d1 = A.items[2 * b + 1];
   const int k0 = d0.x, K = d0.y, S = d0.z;
-  if (K < 0) return;  // the whole workgroup, before any barrier
-  const int64_t prow = d0.w, r0 = d1.x, qb = d1.y;
+  const int64_t r0 = d1.x, qb = d1.y;
   const int ns = (S + 63) >> 6;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x, wc = min(w, ns - 1);
   const bool wave_on = w < ns;
   // LDS: two tile buffers of nthr + 1 rows (row nthr stays zero: the padding slots' row;
-  // the direct loads fill rows [0, nthr)), then the item's tile table (first X row,
-  // 16-byte pieces) and its slice offsets (64-slot units from the item's first slot)
+  // the direct loads fill rows [0, nthr)), then the block's tile table (first X row,
+  // 16-byte pieces) and its slice offsets (64-slot units from the block's first slot)
   const int zrow = A.nthr;
   V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(zrow + 1) * VPR;
   uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf1 + (size_t)(zrow + 1) * VPR);
@@ -269,14 +247,14 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
                                        0);
   };
   auto dma_tile = [&](int kk, V *dst) { dma(mt_c0[kk], mt_np[kk], dst); };
-  // slots [j0, j0 + SELL_PF) of this lane's panel for step kk, read unconditionally (the
-  // slot arrays carry SELL_PF * 64 slots of slack); slots past the panel's end are turned
-  // into (zero row, 0) with masks rather than selects, so the compiler keeps every load
-  // unconditional and never waits for the next tile's loads at a branch
+  // the panel of this wavefront's slice for step kk (the same for the whole wavefront:
+  // held in scalar registers); slots [j0, j0 + SELL_PF) of a lane are read unconditionally
+  // (the slot arrays carry SELL_PF * 64 slots of slack), so the compiler never makes a
+  // load wait for the next tile's loads at a branch
   auto slice_of = [&](int kk, int64_t &base, int &L) {
-    const int o = mt_off[kk * ns + wc];
+    const int o = __builtin_amdgcn_readfirstlane((int)mt_off[kk * ns + wc]);
     base = slot0 + (int64_t)o * 64 + lane;
-    L = (int)mt_off[kk * ns + wc + 1] - o;
+    L = __builtin_amdgcn_readfirstlane((int)mt_off[kk * ns + wc + 1]) - o;
   };
   auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[SELL_PF], T (&h)[SELL_PF]) {
     (void)L;
@@ -312,14 +290,12 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   T h0s[SELL_PF], h1s[SELL_PF], hx[SELL_PF];
   int64_t base0 = 0, base1 = 0;
   int L0 = 0, L1 = 0;
-  // slots j0 + j past the panel's end L act as (zero row, 0), applied here at the use and
-  // not at the load, so the loads stay in flight until the group is reached
+  // slots j0 + j past the panel's end L are skipped here at the use (a branch uniform over
+  // the wavefront), not at the load, so the loads stay in flight until the group is reached
   auto group = [&](const V *cur, const uint32_t (&c)[SELL_PF], const T (&h)[SELL_PF], int j0, int L) {
 #pragma unroll
-    for (int j = 0; j < SELL_PF; ++j) {
-      const bool in = j0 + j < L;
-      fma_row(cur, in ? c[j] : (uint32_t)zrow, in ? h[j] : (T)0);
-    }
+    for (int j = 0; j < SELL_PF; ++j)
+      if (j0 + j < L) fma_row(cur, c[j], h[j]);
   };
   auto step = [&](int kk, const V *cur, V *nxt, uint32_t (&ca)[SELL_PF], T (&ha)[SELL_PF], int64_t base_a, int La,
                   uint32_t (&cb)[SELL_PF], T (&hb)[SELL_PF], int64_t &base_b, int &Lb) {
@@ -352,7 +328,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     // barrier after them also publishes the table
     const int64_t q0 = A.sl_base[qb + wc];
     base0 = q0 + lane;
-    L0 = (int)((A.sl_base[qb + wc + 1] - q0) >> 6);
+    L0 = __builtin_amdgcn_readfirstlane((int)((A.sl_base[qb + wc + 1] - q0) >> 6));
     load_slots(base0, L0, 0, c0s, h0s);
     dma(A.pair_c0[k0], A.pair_rows[k0] * VPR, buf0);
     __builtin_amdgcn_s_waitcnt(0x0f70);
@@ -362,17 +338,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
       if (kk + 1 < K) step(kk + 1, buf1, buf0, c1s, h1s, base1, L1, c0s, h0s, base0, L0);
     }
   }
-  if (tid < S && prow >= 0) {
-    // partial rows: whole pieces (the columns past m are summed but never written to Y)
-    V *y = reinterpret_cast<V *>(P + (prow + tid) * LD);
-#pragma unroll
-    for (int p = 0; p < VPR; ++p) {
-      V u;
-#pragma unroll
-      for (int i = 0; i < VW; ++i) u[i] = acc[p * VW + i];
-      y[rot[p]] = u;
-    }
-  } else if (tid < S) {
+  if (tid < S) {
     T *y = Y + (r0 + tid) * LD;
 #pragma unroll
     for (int p = 0; p < VPR; ++p) {
@@ -391,33 +357,6 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   }
 }
 
-// Y rows of the split row blocks: the block's partial rows summed in slab order, one
-// workgroup per block, one thread per 16-byte piece of a row
-template <typename T, int LD>
-__global__ __launch_bounds__(256) void k_sell_reduce(const int64_t *__restrict__ boff, const int4 *__restrict__ red,
-                                                     const T *__restrict__ P, T *__restrict__ Y, int m) {
-  typedef typename V16t<T>::type V;
-  constexpr int VPR = (int)(LD * sizeof(T) / 16);
-  constexpr int VW = (int)(16 / sizeof(T));
-  const int4 d = red[blockIdx.x];
-  const int64_t r0 = boff[d.x];
-  const int S = (int)(boff[d.x + 1] - r0);
-  const V *src = reinterpret_cast<const V *>(P + (int64_t)d.y * LD);
-  for (int i = threadIdx.x; i < S * VPR; i += blockDim.x) {
-    const int r = i / VPR, p = i - r * VPR;
-    V s = src[i];
-    for (int c = 1; c < d.z; ++c) s += src[(int64_t)c * S * VPR + i];
-    T *y = Y + (r0 + r) * LD;
-    if ((p + 1) * VW <= m) {
-      reinterpret_cast<V *>(y)[p] = s;
-    } else {
-#pragma unroll
-      for (int e = 0; e < VW; ++e)
-        if (p * VW + e < m) y[p * VW + e] = s[e];
-    }
-  }
-}
-
 // ---- host side ------------------------------------------------------------------------
 static std::mutex g_sell_mu;
 
@@ -428,9 +367,6 @@ void free_sell(Op &op) {
                   (void *)s->sl_base, (void *)s->cols, (void *)s->v32, (void *)s->v64, (void *)s->pair_c0,
                   (void *)s->pair_rows, (void *)s->items})
     dfree(p);
-  for (auto &kv : s->slabs)
-    for (void *p : {(void *)kv.second.items, (void *)kv.second.red}) dfree(p);
-  for (auto &kv : s->part) dfree(kv.second.first);
   delete s;
   op.sell = nullptr;
 }
@@ -561,34 +497,22 @@ static int build_sell(Op &op, hipStream_t st) {
   dfree(blk_of);
   dfree(cnt);
   dfree(len);
-  // host copies for the column-slab plans
-  s->h_boff = hb;
-  s->h_nbr_start = hs;
-  s->h_nbr_tile.resize(std::max<int64_t>(s->K, 1));
-  HIPCHK(hipMemcpy(s->h_nbr_tile.data(), s->nbr_tile, sizeof(int32_t) * s->K, hipMemcpyDeviceToHost));
-  s->h_pair_slots.assign(std::max<int64_t>(s->K, 1), 0);
-  for (int b = 0; b < nblk; ++b) {
-    const int ns = (int)((hb[b + 1] - hb[b] + 63) / 64);
-    for (int k = hs[b]; k < hs[b + 1]; ++k) {
-      int64_t c = 0;
-      for (int w = 0; w < ns; ++w) c += hl[hq[b] + (int64_t)(k - hs[b]) * ns + w];
-      s->h_pair_slots[k] = (int32_t)c;
-    }
-  }
-  s->h_qstart = hq;
+  // the column tile of each pair, on the host for the item table (and the statistics)
+  std::vector<int32_t> ht(std::max<int64_t>(s->K, 1));
+  HIPCHK(hipMemcpy(ht.data(), s->nbr_tile, sizeof(int32_t) * s->K, hipMemcpyDeviceToHost));
   // the product's per-pair tile table and whole-block items (rows and slices held as int32)
   if (Q >= INT32_MAX || n >= INT32_MAX) return bail(fail(COSMO_EUNSUP, "tiled layout: too many slices"));
   {
     std::vector<uint32_t> pc(std::max<int64_t>(s->K, 1), 0);
     std::vector<int32_t> pr(std::max<int64_t>(s->K, 1), 0);
     for (int64_t k = 0; k < s->K; ++k) {
-      const int t = s->h_nbr_tile[k];
+      const int t = ht[k];
       pc[k] = (uint32_t)hb[t];
       pr[k] = (int32_t)(hb[t + 1] - hb[t]);
     }
     std::vector<int4> it(2 * (size_t)nblk);
     for (int b = 0; b < nblk; ++b) {
-      it[2 * b] = make_int4(hs[b], hs[b + 1] - hs[b], (int)(hb[b + 1] - hb[b]), -1);
+      it[2 * b] = make_int4(hs[b], hs[b + 1] - hs[b], (int)(hb[b + 1] - hb[b]), 0);
       it[2 * b + 1] = make_int4((int)hb[b], (int)hq[b], 0, 0);
     }
     if (dmalloc(&s->pair_c0, sizeof(uint32_t) * pc.size()) || dmalloc(&s->pair_rows, sizeof(int32_t) * pr.size()) ||
@@ -603,7 +527,6 @@ static int build_sell(Op &op, hipStream_t st) {
              (size_t)E * 2 + 8 * (size_t)s->K + 32 * (size_t)nblk;
   if (g_sell_stats) {
     // X rows one product stages: every (row block, column tile) pair loads the tile's rows
-    const std::vector<int32_t> &ht = s->h_nbr_tile;
     int64_t staged = 0;
     for (int64_t t = 0; t < s->K; ++t) staged += hb[ht[t] + 1] - hb[ht[t]];
     // the same if g neighbouring row blocks staged the union of their tiles once
@@ -666,158 +589,6 @@ static int sell_values(Op &op, bool f32, hipStream_t st) {
   return COSMO_OK;
 }
 
-// The column-slab plan for S slabs (built once per slab count and kept).  Every item and
-// partial row range is checked against the layout here, before it reaches the GPU.
-static int sell_plan(Sell &s, int S, hipStream_t st, const SellSlabs **out) {
-  auto found = s.slabs.find(S);
-  if (found != s.slabs.end()) return *out = &found->second, COSMO_OK;
-  const int nblk = s.nblk;
-  const std::vector<int64_t> &hb = s.h_boff;
-  const std::vector<int32_t> &hs = s.h_nbr_start, &ht = s.h_nbr_tile;
-  // weight of a column tile: the bytes its pairs move at k = 16 FP32 (64-byte X rows
-  // staged, 6-byte slots)
-  std::vector<double> wt(nblk, 0.0);
-  for (int b = 0; b < nblk; ++b)
-    for (int k = hs[b]; k < hs[b + 1]; ++k) {
-      const int t = ht[k];
-      wt[t] += 64.0 * (double)(hb[t + 1] - hb[t]) + 384.0 * (double)s.h_pair_slots[k];
-    }
-  double W = 0.0;
-  for (double x : wt) W += x;
-  std::vector<int> slab_of(nblk, 0);
-  double cw = 0.0;
-  for (int t = 0; t < nblk; ++t) {
-    slab_of[t] = W > 0.0 ? std::min(S - 1, (int)((double)S * (cw + 0.5 * wt[t]) / W)) : 0;
-    cw += wt[t];
-  }
-  // runs of a row block's tiles inside one slab (the tiles ascend, so each slab's are
-  // consecutive), in row-block order; split blocks get consecutive partial row sets
-  struct Run {
-    int s, b, kb, ke;
-    int64_t prow;
-  };
-  std::vector<Run> runs;
-  std::vector<int4> red;
-  int64_t prows = 0;
-  for (int b = 0; b < nblk; ++b) {
-    const size_t first = runs.size();
-    const int nt = hs[b + 1] - hs[b];
-    if (nt == 0) runs.push_back({slab_of[b], b, 0, 0, -1});  // no entries: its Y rows are zero
-    for (int k = 0; k < nt;) {
-      const int sl = slab_of[ht[hs[b] + k]];
-      int e = k + 1;
-      while (e < nt && slab_of[ht[hs[b] + e]] == sl) ++e;
-      runs.push_back({sl, b, k, e, -1});
-      k = e;
-    }
-    const int c = (int)(runs.size() - first);
-    if (c > 1) {
-      const int64_t rows = hb[b + 1] - hb[b];
-      red.push_back(make_int4(b, (int)prows, c, 0));
-      for (int i = 0; i < c; ++i) runs[first + i].prow = prows + i * rows;
-      prows += c * rows;
-      if (prows > INT32_MAX) return fail(COSMO_EUNSUP, "tiled layout: too many partial rows for the column slabs");
-    }
-  }
-  // XCD-major, then slab, then row block (the stable sort keeps the row-block order)
-  std::stable_sort(runs.begin(), runs.end(), [](const Run &a, const Run &c) {
-    return a.s % 8 != c.s % 8 ? a.s % 8 < c.s % 8 : a.s < c.s;
-  });
-  int cnt[8] = {}, start[8] = {};
-  for (const Run &r : runs) ++cnt[r.s % 8];
-  int maxc = 0;
-  for (int x = 0; x < 8; ++x) {
-    start[x] = x ? start[x - 1] + cnt[x - 1] : 0;
-    maxc = std::max(maxc, cnt[x]);
-  }
-  if ((int64_t)maxc * 8 > INT32_MAX) return fail(COSMO_EUNSUP, "tiled layout: too many column-slab items");
-  SellSlabs p;
-  p.S = S;
-  p.nitems = (int)runs.size();
-  p.grid = 8 * maxc;
-  p.nred = (int)red.size();
-  p.prows = prows;
-  std::vector<int4> items(runs.size());
-  std::vector<int32_t> order(p.grid, -1);
-  for (size_t i = 0; i < runs.size(); ++i) items[i] = make_int4(runs[i].b, runs[i].kb, runs[i].ke, (int)runs[i].prow);
-  for (int x = 0; x < 8; ++x)
-    for (int j = 0; j < cnt[x]; ++j) order[(size_t)j * 8 + x] = start[x] + j;
-  // checks: every (row block, tile) pair in exactly one item, item ranges inside their
-  // block, partial rows inside the buffer, each workgroup's item on its slab's XCD
-  std::vector<int32_t> cover(std::max<int64_t>(s.K, 1), 0);
-  std::vector<int32_t> written(nblk, 0);  // each row block's Y rows: one direct item or one reduction
-  for (size_t i = 0; i < items.size(); ++i) {
-    const int4 d = items[i];
-    if (d.x < 0 || d.x >= nblk) return fail(COSMO_EINVAL, "tiled layout: bad column-slab item");
-    const int nt = hs[d.x + 1] - hs[d.x];
-    const int64_t rows = hb[d.x + 1] - hb[d.x];
-    const bool empty = nt == 0 && d.y == 0 && d.z == 0;
-    if (!(empty || (d.y >= 0 && d.y < d.z && d.z <= nt)) || d.w < -1 || (d.w >= 0 && d.w + rows > prows))
-      return fail(COSMO_EINVAL, "tiled layout: bad column-slab item");
-    for (int k = d.y; k < d.z; ++k) ++cover[hs[d.x] + k];
-    if (d.w < 0) ++written[d.x];
-  }
-  for (int64_t k = 0; k < s.K; ++k)
-    if (cover[k] != 1) return fail(COSMO_EINVAL, "tiled layout: column slabs do not cover the pairs once");
-  for (const int4 &d : red) {
-    if (d.x < 0 || d.x >= nblk || d.y < 0 || d.z < 2 || d.y + (int64_t)d.z * (hb[d.x + 1] - hb[d.x]) > prows)
-      return fail(COSMO_EINVAL, "tiled layout: bad column-slab reduction");
-    ++written[d.x];
-  }
-  for (int b = 0; b < nblk; ++b)
-    if (written[b] != 1) return fail(COSMO_EINVAL, "tiled layout: a row block's Y rows are not written once");
-  for (int i = 0; i < p.grid; ++i)
-    if (order[i] >= p.nitems || (order[i] >= 0 && runs[order[i]].s % 8 != i % 8))
-      return fail(COSMO_EINVAL, "tiled layout: bad column-slab order");
-  // the product's records in dispatch order, from the checked runs
-  std::vector<int4> rec(2 * (size_t)p.grid);
-  for (int i = 0; i < p.grid; ++i) {
-    if (order[i] < 0) {
-      rec[2 * i] = make_int4(0, -1, 0, -1);
-      rec[2 * i + 1] = make_int4(0, 0, 0, 0);
-      continue;
-    }
-    const Run &r = runs[order[i]];
-    const int rows = (int)(hb[r.b + 1] - hb[r.b]), ns = (rows + 63) / 64;
-    rec[2 * i] = make_int4(hs[r.b] + r.kb, r.ke - r.kb, rows, (int)r.prow);
-    rec[2 * i + 1] = make_int4((int)hb[r.b], (int)(s.h_qstart[r.b] + (int64_t)r.kb * ns), 0, 0);
-  }
-  if (dmalloc(&p.items, sizeof(int4) * std::max<size_t>(rec.size(), 1)) ||
-      dmalloc(&p.red, sizeof(int4) * std::max(p.nred, 1))) {
-    for (void *q : {(void *)p.items, (void *)p.red}) dfree(q);
-    return fail(COSMO_ENOMEM, "tiled layout: column-slab allocation failed");
-  }
-  HIPCHK(hipMemcpyAsync(p.items, rec.data(), sizeof(int4) * rec.size(), hipMemcpyHostToDevice, st));
-  if (p.nred) HIPCHK(hipMemcpyAsync(p.red, red.data(), sizeof(int4) * p.nred, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-  s.bytes += sizeof(int4) * (rec.size() + (size_t)p.nred);
-  if (g_sell_stats)
-    fprintf(stderr, "[sell] %d column slabs: %d items, %d workgroups, %d split blocks, %lld partial rows (%.3f n)\n",
-            S, p.nitems, p.grid, p.nred, (long long)prows, (double)prows / (double)hb[nblk]);
-  *out = &s.slabs.emplace(S, p).first->second;
-  return COSMO_OK;
-}
-
-// the partial-row buffer of this stream, grown to bytes (stream-ordered: the previous
-// buffer is freed after the stream's earlier work)
-static int sell_partials(Sell &s, size_t bytes, hipStream_t st, void **out) {
-  auto &e = s.part[st];
-  if (e.second < bytes) {
-    if (e.first) HIPCHK(hipFreeAsync(e.first, st));
-    e.first = nullptr;
-    s.bytes -= e.second;
-    e.second = 0;
-    if (dmalloc_void(&e.first, bytes, st)) {
-      e.first = nullptr;
-      return fail(COSMO_ENOMEM, "tiled layout: partial-row allocation failed");
-    }
-    e.second = bytes;
-    s.bytes += bytes;
-  }
-  *out = e.first;
-  return COSMO_OK;
-}
-
 // device bytes the tiled layout of an operator with n rows and nnz stored entries can
 // take with both value copies (the padding of the panels stays below 1.7 x on the
 // systems measured): the server's admission counts it beside the rows
@@ -829,13 +600,10 @@ int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
 int g_sell_chunk = 0;  // XCD chunk of row blocks (COSMO_SELL_CHUNK; 0: chosen by timing)
 int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
 int g_sell_stats = 0;  // COSMO_SELL_STATS=1: the layout's sizes on stderr when it is built
-int g_sell_slabs = 0;  // column slabs of the tiled product (COSMO_SELL_SLABS; 0: none)
 
 template <typename T, int LD>
-static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, int chunk, const SellSlabs *sp, T *P,
-                       hipStream_t st) {
-  SellView v{s.maxtiles, s.nwaves * 64, sp ? 0 : 1,        sp ? sp->items : s.items,
-             s.pair_c0,  s.pair_rows,    s.sl_base, s.cols};
+static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, int chunk, hipStream_t st) {
+  SellView v{s.maxtiles, s.nwaves * 64, s.items, s.pair_c0, s.pair_rows, s.sl_base, s.cols};
   const size_t lds = 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(s);
   if (lds > 160 * 1024) return COSMO_EUNSUP;  // tiles and table do not fit one CU's LDS
   auto kern = k_sell_apply<T, LD>;
@@ -844,27 +612,18 @@ static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, in
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_done = true;
   }
-  const int grid = sp ? sp->grid : s.nblk;
-  if (grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk, P);
+  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk);
   HIPCHK(hipGetLastError());
-  if (sp && sp->nred > 0) {
-    hipLaunchKernelGGL((k_sell_reduce<T, LD>), dim3(sp->nred), dim3(256), 0, st, (const int64_t *)s.boff,
-                       (const int4 *)sp->red, (const T *)P, Y, m);
-    HIPCHK(hipGetLastError());
-  }
   return COSMO_OK;
 }
 
 // One product with the XCD chunk of this (precision, pitch): on the first call every chunk
 // of SELL_CHUNKS is timed (twice each) and the fastest kept.  The chunk only changes which
 // XCD runs which row block, never a row's summation, so every launch writes the same Y.
-// With column slabs the items carry their own XCD order and no chunk applies.
 static constexpr int SELL_CHUNKS[] = {1, 2, 4, 8, 16, 32};
 template <typename T, int LD>
-static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, const SellSlabs *sp, T *P,
-                             hipStream_t st) {
-  if (sp) return launch_sell<T, LD>(s, vals, X, Y, m, 1, sp, P, st);
-  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, X, Y, m, g_sell_chunk, nullptr, nullptr, st);
+static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {
+  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, X, Y, m, g_sell_chunk, st);
   const int key = (int)sizeof(T) * 4096 + LD;
   int chunk;
   {
@@ -881,7 +640,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, co
       for (int c : SELL_CHUNKS)
         for (int rep = 0; rep < 2; ++rep) {
           HIPCHK(hipEventRecord(e0, st));
-          const int rc = launch_sell<T, LD>(s, vals, X, Y, m, c, nullptr, nullptr, st);
+          const int rc = launch_sell<T, LD>(s, vals, X, Y, m, c, st);
           if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -902,7 +661,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, co
       return COSMO_OK;  // the timed launches wrote Y
     }
   }
-  return launch_sell<T, LD>(s, vals, X, Y, m, chunk, nullptr, nullptr, st);
+  return launch_sell<T, LD>(s, vals, X, Y, m, chunk, st);
 }
 
 // Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
@@ -915,41 +674,29 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
   if (f32 && !(ld == 4 || ld == 8 || ld == 16)) return COSMO_EUNSUP;
   if (!f32 && !(ld == 2 || ld == 4 || ld == 8 || ld == 16)) return COSMO_EUNSUP;
   if (f32 && !op.csr.v32) return COSMO_EUNSUP;
-  const int S = op.slabs >= 0 ? op.slabs : g_sell_slabs;
-  const SellSlabs *sp = nullptr;
-  void *P = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_sell_mu);
     if (!op.sell) {
       const int rc = build_sell(op, st);
       if (rc) return rc;
     }
-    int rc = sell_values(op, f32, st);
+    const int rc = sell_values(op, f32, st);
     if (rc) return rc;
-    if (S > 0) {
-      Sell &s = *(Sell *)op.sell;
-      rc = sell_plan(s, S, st, &sp);
-      if (rc) return rc;
-      if (sp->prows > 0) {
-        rc = sell_partials(s, (size_t)sp->prows * ld * (f32 ? sizeof(float) : sizeof(double)), st, &P);
-        if (rc) return rc;
-      }
-    }
   }
   Sell &s = *(Sell *)op.sell;
   if (f32) {
     const float *x = (const float *)X;
-    float *y = (float *)Y, *p = (float *)P;
-    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, x, y, m, sp, p, st);
-    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, x, y, m, sp, p, st);
-    return launch_sell_tuned<float, 16>(s, s.v32, x, y, m, sp, p, st);
+    float *y = (float *)Y;
+    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, x, y, m, st);
+    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, x, y, m, st);
+    return launch_sell_tuned<float, 16>(s, s.v32, x, y, m, st);
   }
   const double *x = (const double *)X;
-  double *y = (double *)Y, *p = (double *)P;
-  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, x, y, m, sp, p, st);
-  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, x, y, m, sp, p, st);
-  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, x, y, m, sp, p, st);
-  return launch_sell_tuned<double, 16>(s, s.v64, x, y, m, sp, p, st);
+  double *y = (double *)Y;
+  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, x, y, m, st);
+  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, x, y, m, st);
+  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, x, y, m, st);
+  return launch_sell_tuned<double, 16>(s, s.v64, x, y, m, st);
 }
 
 }  // namespace cosmo

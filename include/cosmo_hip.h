@@ -197,14 +197,6 @@ int cosmo_op_dense_eigen(cosmo_op op, double *evals, double *evecs);
  * a kernel choice of this build; every choice sums each row in ascending column order
  * within its kernel's partial sums and is deterministic. */
 int cosmo_op_set_layout(cosmo_op op, int layout);
-/* Column slabs of the tiled layout: the column tiles cut into `slabs` ranges of equal
- * weight, each owned by one XCD (slab s by XCD s % 8), so the X tiles of a slab are read
- * from that XCD's L2; row blocks spanning several slabs are summed from per-slab partial
- * rows in slab order.  -1 the library default (COSMO_SELL_SLABS, else none), 0 none,
- * 1..4096 that many.  No reference counterpart: a kernel choice of this build; every
- * choice is deterministic, and the partial rows take slabs-dependent device memory kept
- * with the operator (one buffer per stream used). */
-int cosmo_op_set_slabs(cosmo_op op, int slabs);
 /* Last apply/build timings in milliseconds (device events). */
 int cosmo_op_timings(cosmo_op op, double *build_ms, double *last_apply_ms);
 

@@ -108,37 +108,6 @@ def test_tiled_layout_vs_oracle_and_rows(gpu, golden, name):
         assert np.abs(rows.apply(X, gpu.PREC_FP64) - y64).max() <= 1e-12 * scale
 
 
-@pytest.mark.parametrize("name", ["sd3p1n_m1_s7", "fp3p2n_m1_s3", "sd4p4n_s0"])
-def test_tiled_layout_column_slabs(gpu, golden, name):
-    """The tiled product over column slabs (row blocks split over XCD-owned tile ranges,
-    partial rows summed in slab order): FP64 within 1e-12 and FP32 within 2e-5 of the
-    oracle's CSB product for 1, 3, 8 and 32 slabs and every row pitch, the same bits on a
-    repeated call, and within rounding of the unsplit tiled product."""
-    g = golden(name)
-    s = gpu.System.from_xml(g.xml)
-    o = oracle.System(g.xml)
-    rp, c, v = o.h_upper(drop_small=True)
-    whole = gpu.Operator.build(s)
-    whole.set_layout(2)
-    whole.set_slabs(0)
-    split = gpu.Operator.build(s)
-    split.set_layout(2)
-    n = split.n
-    for S in (1, 3, 8, 32):
-        split.set_slabs(S)
-        for m in (1, 3, 8, 13, 16):
-            X = np.random.default_rng(100 * S + m).standard_normal((n, m))
-            ref = oracle.csb_times(n, rp, c, v, X)
-            scale = np.abs(ref).max()
-            y64 = split.apply(X, gpu.PREC_FP64)
-            y32 = split.apply(X, gpu.PREC_FP32)
-            assert np.abs(y64 - ref).max() <= 1e-12 * scale, (S, m)
-            assert np.abs(y32 - ref).max() <= 2e-5 * scale, (S, m)
-            assert np.array_equal(split.apply(X, gpu.PREC_FP64), y64), (S, m)
-            assert np.array_equal(split.apply(X, gpu.PREC_FP32), y32), (S, m)
-            assert np.abs(whole.apply(X, gpu.PREC_FP64) - y64).max() <= 1e-12 * scale, (S, m)
-
-
 def test_tiled_layout_davidson(gpu, golden):
     """Davidson on the tiled product (FP64 and mixed) equals the row kernel's solve."""
     g = golden("sd4p4n_s0")
