@@ -30,6 +30,10 @@ struct Op {
   void *fact = nullptr;        // product-form tables (COSMO_OP_FACTORED)
   void *sell = nullptr;        // tiled layout of the stored rows (built on the first apply)
   std::vector<int64_t> part;   // row partition: first row of each neutron configuration
+  // per neutron configuration of `part`: the index of the first configuration of its proton
+  // sector in a numbering of all proton configurations, so block b's row r holds proton
+  // configuration part_pbase[b] + (r - part[b]) (empty: unknown or a truncated basis)
+  std::vector<int64_t> part_pbase;
   hipStream_t home = nullptr;  // library stream of the creating thread
   bool foreign = false;        // used on another stream: destroy synchronises the device
   int layout = 0;              // stored product: 0 automatic, 1 rows, 2 tiled (sell.hip)
@@ -55,7 +59,7 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
 void free_sell(Op &op);
 size_t sell_bytes(const Op &op);
 int64_t sell_footprint_bound(int64_t n, int64_t nnz);
-extern int g_sell_chunk, g_sell_off, g_sell_stats;
+extern int g_sell_chunk, g_sell_off, g_sell_stats, g_sell_split;
 int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
 int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);

@@ -229,6 +229,10 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
     } else {
       rc = build_stored(*op, *s, flags, st);
       op->part = s->noff;  // row blocks of the tiled product
+      if (!s->truncated()) {
+        op->part_pbase.resize(s->nsector.size());
+        for (size_t b = 0; b < s->nsector.size(); ++b) op->part_pbase[b] = s->psec_start[s->nsector[b]];
+      }
     }
   }
   if (!rc) rc = fetch_diag(*op, st);

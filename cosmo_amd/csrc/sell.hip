@@ -55,7 +55,32 @@ struct Sell {
   int4 *items = nullptr;         // [2 nblk] the item of each row block
   size_t bytes = 0;
   std::map<int, int> chunk;      // (element size, pitch) -> XCD chunk chosen by timing
+  // Proton-spectator part: the entries between two different neutron configurations with
+  // the same proton configuration (neutron two-move terms, and one-move terms whose proton
+  // configuration is unchanged), multiplied on proton-major tiles by k_pm_apply and left
+  // out of the panels above.  A proton-major tile is one proton configuration with every
+  // neutron configuration of its class (the neutron configurations whose sector holds it):
+  // these entries never leave it.  Units: (class, proton rank, rows of the class from i0).
+  bool split = false;
+  int pm_units = 0, pm_maxC = 0, pm_nwaves = 0;
+  int64_t pm_Q = 0, pm_E = 0, pm_Ealloc = 0;
+  uint64_t *rid = nullptr;       // [n] (neutron block << 32) | proton configuration of each row
+  int32_t *cls_local = nullptr;  // [neutron blocks] index of the block within its class
+  uint32_t *pm_rows = nullptr;   // class members' first rows, class after class
+  int4 *pm_items = nullptr;      // [2 units]: (first member, members C, proton rank, i0), (rows, first slice)
+  int64_t *pm_base = nullptr;    // [pm_Q + 1] first slot of each slice (a multiple of 64)
+  uint16_t *pm_cols = nullptr;   // [pm_Ealloc] 1 + member index of the column (0: the zero row)
+  float *pm_v32 = nullptr;
+  double *pm_v64 = nullptr;
 };
+
+// an entry (r, c) of the proton-spectator part: the same proton configuration, another
+// neutron configuration
+__device__ __forceinline__ bool pm_entry(const uint64_t *__restrict__ rid, uint64_t ra, uint32_t c) {
+  if (!rid) return false;
+  const uint64_t rb = rid[c];
+  return (uint32_t)ra == (uint32_t)rb && (ra >> 32) != (rb >> 32);
+}
 
 // A row block's item is two int4: (first pair, tiles, rows, 0) and (first row, first
 // slice, 0, 0); pairs are the (row block, column tile) pairs in block-major order.
@@ -77,15 +102,28 @@ __global__ __launch_bounds__(256) void k_sell_tiles(int pass, const int64_t *__r
                                                     const int32_t *__restrict__ blk_of, int nblk,
                                                     int32_t *__restrict__ count,
                                                     const int32_t *__restrict__ nbr_start,
-                                                    int32_t *__restrict__ nbr_tile) {
+                                                    int32_t *__restrict__ nbr_tile,
+                                                    const uint64_t *__restrict__ rid) {
   extern __shared__ uint32_t bm[];
   const int b = blockIdx.x, nw = (nblk + 31) >> 5;
   for (int w = threadIdx.x; w < nw; w += blockDim.x) bm[w] = 0u;
   __syncthreads();
-  const int64_t e0 = rowptr[boff[b]], e1 = rowptr[boff[b + 1]];
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int t = blk_of[cols[e]];
-    atomicOr(&bm[t >> 5], 1u << (t & 31));
+  if (!rid) {
+    const int64_t e0 = rowptr[boff[b]], e1 = rowptr[boff[b + 1]];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+      const int t = blk_of[cols[e]];
+      atomicOr(&bm[t >> 5], 1u << (t & 31));
+    }
+  } else {  // one thread per row, the proton-spectator entries left out
+    for (int64_t r = boff[b] + threadIdx.x; r < boff[b + 1]; r += blockDim.x) {
+      const uint64_t ra = rid[r];
+      for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+        const uint32_t c = cols[e];
+        if (pm_entry(rid, ra, c)) continue;
+        const int t = blk_of[c];
+        atomicOr(&bm[t >> 5], 1u << (t & 31));
+      }
+    }
   }
   __syncthreads();
   if (pass == 0) {
@@ -112,7 +150,8 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
     const uint32_t *__restrict__ cols, const double *__restrict__ v64, const float *__restrict__ v32,
     const int32_t *__restrict__ nbr_start, const int32_t *__restrict__ nbr_tile,
     const int64_t *__restrict__ qstart, int64_t *__restrict__ len, const int64_t *__restrict__ sl_base,
-    uint16_t *__restrict__ scol, double *__restrict__ s64, float *__restrict__ s32) {
+    uint16_t *__restrict__ scol, double *__restrict__ s64, float *__restrict__ s32,
+    const uint64_t *__restrict__ rid) {
   const int b = blockIdx.x;
   const int64_t r0 = boff[b];
   const int S = (int)(boff[b + 1] - r0), ns = (S + 63) >> 6;
@@ -121,6 +160,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
   const bool on = rho < S;
   int64_t e = on ? rowptr[r0 + rho] : 0;
   const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
+  const uint64_t ra = (on && rid) ? rid[r0 + rho] : 0;
   const int k0 = nbr_start[b], k1 = nbr_start[b + 1];
   for (int k = k0; k < k1; ++k) {
     const int t = nbr_tile[k];
@@ -128,19 +168,59 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
     const int64_t q = qstart[b] + (int64_t)(k - k0) * ns + w;
     int c = 0;
     if (pass == 0) {
-      while (e < ee && cols[e] < ce) ++e, ++c;
+      for (; e < ee && cols[e] < ce; ++e)
+        if (!pm_entry(rid, ra, cols[e])) ++c;
       for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
       if (lane == 0) len[q] = c;
     } else {
       const int64_t base = sl_base[q] + lane;
-      while (e < ee && cols[e] < ce) {
+      for (; e < ee && cols[e] < ce; ++e) {
+        if (pm_entry(rid, ra, cols[e])) continue;
         const int64_t p = base + (int64_t)c * 64;
         if (scol) scol[p] = (uint16_t)(cols[e] - cs);
         if (s64) s64[p] = v64[e];
         if (s32) s32[p] = v32[e];
-        ++e;
         ++c;
       }
+    }
+  }
+}
+
+// the proton-spectator part's panels: one workgroup per unit, one lane per row of the
+// unit (member i0 + lane of the class at the unit's proton rank); pass 0 writes each
+// slice's longest row to len[q], pass 1 the entries (in column order) into their slots
+__global__ __launch_bounds__(SELL_MAXROWS) void k_pm_slots(
+    int pass, const int4 *__restrict__ items, const uint32_t *__restrict__ pm_rows, const uint64_t *__restrict__ rid,
+    const int32_t *__restrict__ cls_local, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+    const double *__restrict__ v64, const float *__restrict__ v32, int64_t *__restrict__ len,
+    const int64_t *__restrict__ base, uint16_t *__restrict__ scol, double *__restrict__ s64,
+    float *__restrict__ s32) {
+  const int4 u0 = items[2 * blockIdx.x], u1 = items[2 * blockIdx.x + 1];
+  const int S = u1.x, ns = (S + 63) >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (w >= ns) return;
+  const bool on = tid < S;
+  const int64_t r = on ? (int64_t)pm_rows[u0.x + u0.w + tid] + u0.z : 0;
+  int64_t e = on ? rowptr[r] : 0;
+  const int64_t ee = on ? rowptr[r + 1] : 0;
+  const uint64_t ra = on ? rid[r] : 0;
+  const int64_t q = (int64_t)u1.y + w;
+  int c = 0;
+  if (pass == 0) {
+    for (; e < ee; ++e)
+      if (pm_entry(rid, ra, cols[e])) ++c;
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off));
+    if (lane == 0) len[q] = c;
+  } else {
+    const int64_t b0 = base[q] + lane;
+    for (; e < ee; ++e) {
+      const uint32_t col = cols[e];
+      if (!pm_entry(rid, ra, col)) continue;
+      const int64_t p = b0 + (int64_t)c * 64;
+      if (scol) scol[p] = (uint16_t)(1 + cls_local[rid[col] >> 32]);
+      if (s64) s64[p] = v64[e];
+      if (s32) s32[p] = v32[e];
+      ++c;
     }
   }
 }
@@ -152,7 +232,8 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *_
                                                                const uint32_t *__restrict__ cols,
                                                                const int32_t *__restrict__ nbr_start,
                                                                const int32_t *__restrict__ nbr_tile,
-                                                               unsigned long long *__restrict__ out) {
+                                                               unsigned long long *__restrict__ out,
+                                                               const uint64_t *__restrict__ rid) {
   constexpr int NW = SELL_MAXROWS / 32;
   __shared__ uint32_t bm[NW];
   __shared__ int tot;
@@ -161,13 +242,15 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *_
   const bool on = rho < (int)(boff[b + 1] - r0);
   int64_t e = on ? rowptr[r0 + rho] : 0;
   const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
+  const uint64_t ra = (on && rid) ? rid[r0 + rho] : 0;
   if (rho == 0) tot = 0;
   for (int k = nbr_start[b]; k < nbr_start[b + 1]; ++k) {
     if (rho < NW) bm[rho] = 0u;
     __syncthreads();
     const int t = nbr_tile[k];
     const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
-    for (; e < ee && cols[e] < ce; ++e) atomicOr(&bm[(cols[e] - cs) >> 5], 1u << ((cols[e] - cs) & 31));
+    for (; e < ee && cols[e] < ce; ++e)
+      if (!pm_entry(rid, ra, cols[e])) atomicOr(&bm[(cols[e] - cs) >> 5], 1u << ((cols[e] - cs) & 31));
     __syncthreads();
     if (rho < NW) atomicAdd(&tot, __popc(bm[rho]));
     __syncthreads();
@@ -203,7 +286,7 @@ __device__ __forceinline__ int sell_block(int chunk) {
 template <typename T, int LD>
 __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals,
                                                             const T *__restrict__ X, T *__restrict__ Y, int m,
-                                                            int chunk) {
+                                                            int chunk, int accum) {
   typedef typename V16t<T>::type V;
   constexpr int VPR = (int)(LD * sizeof(T) / 16);  // 16-byte pieces per X row
   constexpr int VW = (int)(16 / sizeof(T));        // elements per piece
@@ -231,9 +314,24 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   }
   for (int i = tid; i <= K * ns; i += nthr) mt_off[i] = (uint16_t)((A.sl_base[qb + i] - slot0) >> 6);
   if (tid < 2 * VPR) buf0[(size_t)(tid / VPR) * (zrow + 1) * VPR + (size_t)zrow * VPR + tid % VPR] = V{};
+  // instruction p of an entry reads piece (p + lane) % VPR of its X row into accumulator
+  // block p: lanes on random rows then hit all LDS banks (a plain piece-p read from rows
+  // 64 bytes apart reaches a quarter of them); the rotation is undone when Y is written
+  int rot[VPR];
+#pragma unroll
+  for (int p = 0; p < VPR; ++p) rot[p] = (p + lane) & (VPR - 1);
   T acc[LD];
 #pragma unroll
   for (int i = 0; i < LD; ++i) acc[i] = (T)0;
+  if (accum && tid < S) {  // Y already holds the proton-spectator part of the row
+    const V *yr = reinterpret_cast<const V *>(Y + (r0 + tid) * LD);
+#pragma unroll
+    for (int p = 0; p < VPR; ++p) {
+      const V u = yr[rot[p]];
+#pragma unroll
+      for (int i = 0; i < VW; ++i) acc[p * VW + i] = u[i];
+    }
+  }
   // tile loads straight into LDS (global_load_lds_dwordx4: lane i of a wavefront writes
   // 16 bytes at the wavefront's LDS base + 16 i): piece idx = tid + p * nthr of the tile's
   // contiguous X rows goes to LDS piece idx; pieces past the tile load the tile's last
@@ -265,12 +363,6 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
       h[j] = vals[p];
     }
   };
-  // instruction p of an entry reads piece (p + lane) % VPR of its X row into accumulator
-  // block p: lanes on random rows then hit all LDS banks (a plain piece-p read from rows
-  // 64 bytes apart reaches a quarter of them); the rotation is undone when Y is written
-  int rot[VPR];
-#pragma unroll
-  for (int p = 0; p < VPR; ++p) rot[p] = (p + lane) & (VPR - 1);
   auto fma_row = [&](const V *tile, uint32_t c, T h) {
     const V *xr = tile + c * VPR;
 #pragma unroll
@@ -357,6 +449,110 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   }
 }
 
+// Proton-spectator part of Y = H X: one workgroup per unit (a proton configuration and
+// up to SELL_MAXROWS neutron configurations of its class), one lane per row.  The X rows
+// of the whole proton-major tile (every member of the class at the unit's proton rank:
+// one 16-byte-aligned row per member, scattered through X) are gathered straight into LDS
+// (global_load_lds) behind a zero row; each lane then runs its ELL panel (16-bit member
+// index + 1, value) against them and writes its whole Y row.  Every row of the operator
+// belongs to exactly one unit, so this launch writes every row of Y once.
+struct PmView {
+  const int4 *items;
+  const uint32_t *rows;
+  const int64_t *base;
+  const uint16_t *cols;
+};
+
+template <typename T, int LD>
+__global__ __launch_bounds__(SELL_MAXROWS) void k_pm_apply(PmView P, const T *__restrict__ vals,
+                                                          const T *__restrict__ X, T *__restrict__ Y, int m,
+                                                          int tile_pieces) {
+  typedef typename V16t<T>::type V;
+  constexpr int VPR = (int)(LD * sizeof(T) / 16);
+  constexpr int VW = (int)(16 / sizeof(T));
+  constexpr int PF = 4;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int u = sell_block(8);
+  const int4 u0 = P.items[2 * u], u1 = P.items[2 * u + 1];
+  const int C = u0.y, i0 = u0.w, S = u1.x;
+  const uint32_t rho = (uint32_t)u0.z;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x;
+  const int ns = (S + 63) >> 6, wc = min(w, ns - 1);
+  // LDS: the zero row, then the tile (tile_pieces pieces: C rows rounded up to whole
+  // passes of the workgroup), then the members' X rows
+  V *tile = reinterpret_cast<V *>(lds_raw);
+  uint32_t *xrow = reinterpret_cast<uint32_t *>(tile + VPR + tile_pieces);
+  for (int j = tid; j < C; j += nthr) xrow[j] = P.rows[u0.x + j] + rho;
+  if (tid < VPR) tile[tid] = V{};
+  // this lane's slice: slots base + 64 j, j < L; the first group is loaded before the tile
+  const int64_t sb = P.base[u1.y + wc];
+  const int L = __builtin_amdgcn_readfirstlane((int)((P.base[u1.y + wc + 1] - sb) >> 6));
+  const int64_t base = sb + lane;
+  uint32_t ca[PF], cb[PF];
+  T ha[PF], hb[PF];
+  auto load = [&](int j0, uint32_t (&c)[PF], T (&h)[PF]) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {  // the slot arrays carry PF * 64 slots of slack
+      c[j] = P.cols[base + (int64_t)(j0 + j) * 64];
+      h[j] = vals[base + (int64_t)(j0 + j) * 64];
+    }
+  };
+  load(0, ca, ha);
+  __syncthreads();
+  const int np = C * VPR;
+  for (int b0 = 0; b0 < np; b0 += nthr) {
+    const int idx = min(b0 + tid, np - 1), j = idx / VPR, p = idx - j * VPR;
+    const V *src = reinterpret_cast<const V *>(X + (int64_t)xrow[j] * LD) + p;
+    __builtin_amdgcn_global_load_lds((const void *)src,
+                                     (__attribute__((address_space(3))) void *)(tile + VPR + b0 + w * 64), 16, 0, 0);
+  }
+  int rot[VPR];
+#pragma unroll
+  for (int p = 0; p < VPR; ++p) rot[p] = (p + lane) & (VPR - 1);
+  T acc[LD];
+#pragma unroll
+  for (int i = 0; i < LD; ++i) acc[i] = (T)0;
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the gathered rows have landed
+  __syncthreads();
+  if (w >= ns) return;  // no barrier below
+  auto group = [&](const uint32_t (&c)[PF], const T (&h)[PF], int j0) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+      if (j0 + j < L) {
+        const V *xr = tile + c[j] * VPR;
+#pragma unroll
+        for (int p = 0; p < VPR; ++p) {
+          const V x = xr[rot[p]];
+#pragma unroll
+          for (int i = 0; i < VW; ++i) acc[p * VW + i] += h[j] * x[i];
+        }
+      }
+  };
+  for (int j0 = 0; j0 < L; j0 += 2 * PF) {
+    load(j0 + PF, cb, hb);
+    group(ca, ha, j0);
+    if (j0 + PF >= L) break;
+    load(j0 + 2 * PF, ca, ha);
+    group(cb, hb, j0 + PF);
+  }
+  if (tid < S) {
+    T *y = Y + (int64_t)xrow[i0 + tid] * LD;
+#pragma unroll
+    for (int p = 0; p < VPR; ++p) {
+      if ((rot[p] + 1) * VW <= m) {
+        V v;
+#pragma unroll
+        for (int i = 0; i < VW; ++i) v[i] = acc[p * VW + i];
+        reinterpret_cast<V *>(y)[rot[p]] = v;
+      } else {
+#pragma unroll
+        for (int i = 0; i < VW; ++i)
+          if (rot[p] * VW + i < m) y[rot[p] * VW + i] = acc[p * VW + i];
+      }
+    }
+  }
+}
+
 // ---- host side ------------------------------------------------------------------------
 static std::mutex g_sell_mu;
 
@@ -365,7 +561,9 @@ void free_sell(Op &op) {
   if (!s) return;
   for (void *p : {(void *)s->boff, (void *)s->nbr_start, (void *)s->nbr_tile, (void *)s->qstart,
                   (void *)s->sl_base, (void *)s->cols, (void *)s->v32, (void *)s->v64, (void *)s->pair_c0,
-                  (void *)s->pair_rows, (void *)s->items})
+                  (void *)s->pair_rows, (void *)s->items, (void *)s->rid, (void *)s->cls_local,
+                  (void *)s->pm_rows, (void *)s->pm_items, (void *)s->pm_base, (void *)s->pm_cols,
+                  (void *)s->pm_v32, (void *)s->pm_v64})
     dfree(p);
   delete s;
   op.sell = nullptr;
@@ -405,6 +603,117 @@ static size_t sell_meta_bytes(const Sell &s) {
   return 8 * (size_t)s.maxtiles + 2 * ((size_t)s.maxtiles * s.nwaves + 1) + 16;
 }
 
+#define PM_PF 4  // slots per group in k_pm_apply (the slot arrays carry 4 groups of slack)
+
+// LDS bytes of k_pm_apply at a row pitch of `row` bytes: zero row, tile, member rows
+static size_t pm_lds_bytes(const Sell &s, size_t row) {
+  const int64_t vpr = (int64_t)row / 16, nthr = s.pm_nwaves * 64;
+  const int64_t pieces = (s.pm_maxC * vpr + nthr - 1) / nthr * nthr;
+  return 16 * (size_t)(vpr + pieces) + 4 * (size_t)s.pm_maxC;
+}
+
+// The proton-spectator split of an operator built from a full basis: classes are the
+// neutron blocks that share a proton sector (so rows of a class line up proton rank by
+// proton rank).  Fills s.rid, s.cls_local, s.pm_rows, s.pm_items and the unit counts;
+// returns false (no split) when the operator carries no basis structure, a class's blocks
+// differ in length, or a tile at the widest pitch would not fit LDS.
+static bool plan_pm(Op &op, Sell &s, hipStream_t st, int &rc) {
+  rc = COSMO_OK;
+  const std::vector<int64_t> &P = op.part;
+  const std::vector<int64_t> &B = op.part_pbase;
+  if (!g_sell_split || P.size() < 2 || B.size() + 1 != P.size() || P.back() != op.n) return false;
+  const int nb = (int)B.size();
+  std::map<int64_t, std::vector<int>> cls;
+  for (int v = 0; v < nb; ++v) cls[B[v]].push_back(v);
+  int cmax = 0;
+  for (auto &kv : cls) {
+    const int64_t S = P[kv.second[0] + 1] - P[kv.second[0]];
+    for (int v : kv.second)
+      if (P[v + 1] - P[v] != S) return false;
+    cmax = std::max<int>(cmax, (int)kv.second.size());
+  }
+  if (cmax >= 65535) return false;
+  s.pm_maxC = cmax;
+  s.pm_nwaves = (std::min(cmax, SELL_MAXROWS) + 63) / 64;
+  if (pm_lds_bytes(s, 16 * sizeof(double)) > 160 * 1024) return false;
+  std::vector<uint64_t> hrid(op.n);
+  std::vector<int32_t> loc(nb);
+  std::vector<uint32_t> rows;
+  std::vector<int4> items;
+  int64_t q = 0;
+  for (auto &kv : cls) {
+    const std::vector<int> &mem = kv.second;
+    const int C = (int)mem.size(), first = (int)rows.size();
+    const int64_t S = P[mem[0] + 1] - P[mem[0]];
+    for (int i = 0; i < C; ++i) {
+      const int v = mem[i];
+      loc[v] = i;
+      rows.push_back((uint32_t)P[v]);
+      for (int64_t r = P[v]; r < P[v + 1]; ++r) hrid[r] = ((uint64_t)v << 32) | (uint64_t)(B[v] + (r - P[v]));
+    }
+    for (int64_t rho = 0; rho < S; ++rho)
+      for (int i0 = 0; i0 < C; i0 += SELL_MAXROWS) {
+        const int rows_u = std::min(SELL_MAXROWS, C - i0);
+        items.push_back(make_int4(first, C, (int)rho, i0));
+        items.push_back(make_int4(rows_u, (int)q, 0, 0));
+        q += (rows_u + 63) / 64;
+        if (q >= INT32_MAX || rho >= INT32_MAX) return false;
+      }
+  }
+  s.pm_units = (int)(items.size() / 2);
+  s.pm_Q = q;
+  if (dmalloc(&s.rid, sizeof(uint64_t) * op.n) || dmalloc(&s.cls_local, sizeof(int32_t) * nb) ||
+      dmalloc(&s.pm_rows, sizeof(uint32_t) * rows.size()) || dmalloc(&s.pm_items, sizeof(int4) * items.size())) {
+    rc = fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+    return false;
+  }
+  if (hipMemcpyAsync(s.rid, hrid.data(), sizeof(uint64_t) * op.n, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(s.cls_local, loc.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(s.pm_rows, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(s.pm_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, st) ||
+      hipStreamSynchronize(st)) {
+    rc = fail(COSMO_EHIP, "tiled layout: upload failed");
+    return false;
+  }
+  s.bytes += sizeof(uint64_t) * op.n + sizeof(int32_t) * nb + sizeof(uint32_t) * rows.size() +
+             sizeof(int4) * items.size();
+  return true;
+}
+
+// the proton-spectator panels: lengths, slot offsets, columns
+static int build_pm(const Op &op, Sell &s, hipStream_t st) {
+  const CsrDev &M = op.csr;
+  const int thr = s.pm_nwaves * 64;
+  int64_t *len = nullptr;
+  if (dmalloc(&len, sizeof(int64_t) * std::max<int64_t>(s.pm_Q, 1)) || dmalloc(&s.pm_base, sizeof(int64_t) * (s.pm_Q + 1)))
+    return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+  hipLaunchKernelGGL(k_pm_slots, dim3(s.pm_units), dim3(thr), 0, st, 0, (const int4 *)s.pm_items,
+                     (const uint32_t *)s.pm_rows, (const uint64_t *)s.rid, (const int32_t *)s.cls_local,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, len, (const int64_t *)nullptr, (uint16_t *)nullptr, (double *)nullptr,
+                     (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<int64_t> hl(s.pm_Q), hb(s.pm_Q + 1, 0);
+  HIPCHK(hipMemcpyAsync(hl.data(), len, sizeof(int64_t) * s.pm_Q, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(len);
+  for (int64_t q = 0; q < s.pm_Q; ++q) hb[q + 1] = hb[q] + 64 * hl[q];
+  s.pm_E = hb[s.pm_Q];
+  s.pm_Ealloc = s.pm_E + 4 * PM_PF * 64;
+  if (dmalloc(&s.pm_cols, sizeof(uint16_t) * s.pm_Ealloc)) return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+  HIPCHK(hipMemcpyAsync(s.pm_base, hb.data(), sizeof(int64_t) * (s.pm_Q + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(s.pm_cols, 0, sizeof(uint16_t) * s.pm_Ealloc, st));  // padding: the zero row
+  hipLaunchKernelGGL(k_pm_slots, dim3(s.pm_units), dim3(thr), 0, st, 1, (const int4 *)s.pm_items,
+                     (const uint32_t *)s.pm_rows, (const uint64_t *)s.rid, (const int32_t *)s.cls_local,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, (int64_t *)nullptr, (const int64_t *)s.pm_base, s.pm_cols,
+                     (double *)nullptr, (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  s.bytes += sizeof(int64_t) * (s.pm_Q + 1) + sizeof(uint16_t) * s.pm_Ealloc;
+  return COSMO_OK;
+}
+
 static int build_sell(Op &op, hipStream_t st) {
   const CsrDev &M = op.csr;
   const int64_t n = op.n;
@@ -423,6 +732,13 @@ static int build_sell(Op &op, hipStream_t st) {
     free_sell(op);
     return rc;
   };
+  {
+    int rc = COSMO_OK;
+    s->split = plan_pm(op, *s, st, rc);
+    if (rc) return bail(rc);
+    if (s->split && (rc = build_pm(op, *s, st))) return bail(rc);
+  }
+  const uint64_t *rid = s->split ? s->rid : nullptr;
   std::vector<int32_t> hblk(n);
   for (int b = 0; b < nblk; ++b)
     for (int64_t r = hb[b]; r < hb[b + 1]; ++r) hblk[r] = b;
@@ -436,7 +752,7 @@ static int build_sell(Op &op, hipStream_t st) {
   const size_t bm_bytes = sizeof(uint32_t) * (size_t)((nblk + 31) / 32);
   hipLaunchKernelGGL(k_sell_tiles, dim3(nblk), dim3(256), bm_bytes, st, 0, (const int64_t *)s->boff,
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)blk_of, nblk, cnt,
-                     (const int32_t *)nullptr, (int32_t *)nullptr);
+                     (const int32_t *)nullptr, (int32_t *)nullptr, rid);
   HIPCHK(hipGetLastError());
   std::vector<int32_t> hc(nblk), hs(nblk + 1, 0);
   HIPCHK(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, st));
@@ -458,14 +774,14 @@ static int build_sell(Op &op, hipStream_t st) {
   HIPCHK(hipMemcpyAsync(s->qstart, hq.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_sell_tiles, dim3(nblk), dim3(256), bm_bytes, st, 1, (const int64_t *)s->boff,
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)blk_of, nblk, cnt,
-                     (const int32_t *)s->nbr_start, s->nbr_tile);
+                     (const int32_t *)s->nbr_start, s->nbr_tile, rid);
   HIPCHK(hipGetLastError());
   const int thr = s->nwaves * 64;
   hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 0, (const int64_t *)s->boff,
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
                      (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
                      (const int64_t *)s->qstart, len, (const int64_t *)nullptr, (uint16_t *)nullptr,
-                     (double *)nullptr, (float *)nullptr);
+                     (double *)nullptr, (float *)nullptr, rid);
   HIPCHK(hipGetLastError());
   std::vector<int64_t> hl(Q), hbase(Q + 1, 0);
   HIPCHK(hipMemcpyAsync(hl.data(), len, sizeof(int64_t) * Q, hipMemcpyDeviceToHost, st));
@@ -491,7 +807,7 @@ static int build_sell(Op &op, hipStream_t st) {
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
                      (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
                      (const int64_t *)s->qstart, (int64_t *)nullptr, (const int64_t *)s->sl_base, s->cols,
-                     (double *)nullptr, (float *)nullptr);
+                     (double *)nullptr, (float *)nullptr, rid);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   dfree(blk_of);
@@ -523,7 +839,7 @@ static int build_sell(Op &op, hipStream_t st) {
     HIPCHK(hipMemcpyAsync(s->items, it.data(), sizeof(int4) * it.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
-  s->bytes = sizeof(int64_t) * (nblk + 1) * 2 + sizeof(int32_t) * (nblk + 1 + s->K) + sizeof(int64_t) * (Q + 1) +
+  s->bytes += sizeof(int64_t) * (nblk + 1) * 2 + sizeof(int32_t) * (nblk + 1 + s->K) + sizeof(int64_t) * (Q + 1) +
              (size_t)E * 2 + 8 * (size_t)s->K + 32 * (size_t)nblk;
   if (g_sell_stats) {
     // X rows one product stages: every (row block, column tile) pair loads the tile's rows
@@ -549,7 +865,7 @@ static int build_sell(Op &op, hipStream_t st) {
       HIPCHK(hipMemsetAsync(dd, 0, sizeof(unsigned long long), st));
       hipLaunchKernelGGL(k_sell_distinct, dim3(nblk), dim3(thr), 0, st, (const int64_t *)s->boff,
                          (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)s->nbr_start,
-                         (const int32_t *)s->nbr_tile, dd);
+                         (const int32_t *)s->nbr_tile, dd, rid);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&distinct, dd, sizeof(distinct), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -561,6 +877,9 @@ static int build_sell(Op &op, hipStream_t st) {
             (long long)n, nblk, maxrows, (long long)s->K, (long long)Q, (long long)s->E, (long long)M.nnz,
             M.nnz ? (double)s->E / (double)M.nnz : 0.0, (long long)staged, (double)staged / (double)n, distinct,
             (double)distinct / (double)n);
+    if (s->split)
+      fprintf(stderr, "[sell] proton-spectator part: units=%d (class rows <= %d) slices=%lld slots=%lld\n",
+              s->pm_units, s->pm_maxC, (long long)s->pm_Q, (long long)s->pm_E);
   }
   op.sell = s;
   return COSMO_OK;
@@ -580,12 +899,28 @@ static int sell_values(Op &op, bool f32, hipStream_t st) {
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)M.v64,
                      (const float *)M.v32, (const int32_t *)s.nbr_start, (const int32_t *)s.nbr_tile,
                      (const int64_t *)s.qstart, (int64_t *)nullptr, (const int64_t *)s.sl_base, (uint16_t *)nullptr,
-                     f32 ? (double *)nullptr : (double *)dst, f32 ? (float *)dst : (float *)nullptr);
+                     f32 ? (double *)nullptr : (double *)dst, f32 ? (float *)dst : (float *)nullptr,
+                     s.split ? (const uint64_t *)s.rid : (const uint64_t *)nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   if (f32) s.v32 = (float *)dst;
   else s.v64 = (double *)dst;
   s.bytes += (f32 ? sizeof(float) : sizeof(double)) * (size_t)s.Ealloc;
+  if (!s.split) return COSMO_OK;
+  void *pd = nullptr;
+  const size_t es = f32 ? sizeof(float) : sizeof(double);
+  if (dmalloc_void(&pd, es * s.pm_Ealloc, st)) return fail(COSMO_ENOMEM, "tiled layout: value allocation failed");
+  HIPCHK(hipMemsetAsync(pd, 0, es * s.pm_Ealloc, st));
+  hipLaunchKernelGGL(k_pm_slots, dim3(s.pm_units), dim3(s.pm_nwaves * 64), 0, st, 1, (const int4 *)s.pm_items,
+                     (const uint32_t *)s.pm_rows, (const uint64_t *)s.rid, (const int32_t *)s.cls_local,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)M.v64,
+                     (const float *)M.v32, (int64_t *)nullptr, (const int64_t *)s.pm_base, (uint16_t *)nullptr,
+                     f32 ? (double *)nullptr : (double *)pd, f32 ? (float *)pd : (float *)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  if (f32) s.pm_v32 = (float *)pd;
+  else s.pm_v64 = (double *)pd;
+  s.bytes += es * (size_t)s.pm_Ealloc;
   return COSMO_OK;
 }
 
@@ -600,19 +935,33 @@ int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
 int g_sell_chunk = 0;  // XCD chunk of row blocks (COSMO_SELL_CHUNK; 0: chosen by timing)
 int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
 int g_sell_stats = 0;  // COSMO_SELL_STATS=1: the layout's sizes on stderr when it is built
+int g_sell_split = 1;  // COSMO_SELL_SPLIT=0: the proton-spectator entries stay in the panels
 
+// one product: the proton-spectator part (when split) writes every row of Y, then the
+// panels add the rest of each row to it
 template <typename T, int LD>
-static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, int chunk, hipStream_t st) {
+static int launch_sell(const Sell &s, const T *vals, const T *pm_vals, const T *X, T *Y, int m, int chunk,
+                       hipStream_t st) {
   SellView v{s.maxtiles, s.nwaves * 64, s.items, s.pair_c0, s.pair_rows, s.sl_base, s.cols};
   const size_t lds = 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(s);
   if (lds > 160 * 1024) return COSMO_EUNSUP;  // tiles and table do not fit one CU's LDS
   auto kern = k_sell_apply<T, LD>;
+  auto pkern = k_pm_apply<T, LD>;
   static bool attr_done = false;
   if (!attr_done) {
     (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)pkern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_done = true;
   }
-  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk);
+  if (s.split) {
+    PmView pv{s.pm_items, s.pm_rows, s.pm_base, s.pm_cols};
+    const int vpr = (int)(LD * sizeof(T) / 16), nthr = s.pm_nwaves * 64;
+    const int pieces = (s.pm_maxC * vpr + nthr - 1) / nthr * nthr;
+    hipLaunchKernelGGL(pkern, dim3(s.pm_units), dim3(nthr), pm_lds_bytes(s, LD * sizeof(T)), st, pv, pm_vals, X, Y,
+                       m, pieces);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk, s.split ? 1 : 0);
   HIPCHK(hipGetLastError());
   return COSMO_OK;
 }
@@ -622,8 +971,8 @@ static int launch_sell(const Sell &s, const T *vals, const T *X, T *Y, int m, in
 // XCD runs which row block, never a row's summation, so every launch writes the same Y.
 static constexpr int SELL_CHUNKS[] = {1, 2, 4, 8, 16, 32};
 template <typename T, int LD>
-static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, hipStream_t st) {
-  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, X, Y, m, g_sell_chunk, st);
+static int launch_sell_tuned(Sell &s, const T *vals, const T *pm_vals, const T *X, T *Y, int m, hipStream_t st) {
+  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, g_sell_chunk, st);
   const int key = (int)sizeof(T) * 4096 + LD;
   int chunk;
   {
@@ -640,7 +989,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, hi
       for (int c : SELL_CHUNKS)
         for (int rep = 0; rep < 2; ++rep) {
           HIPCHK(hipEventRecord(e0, st));
-          const int rc = launch_sell<T, LD>(s, vals, X, Y, m, c, st);
+          const int rc = launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, c, st);
           if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -661,7 +1010,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *X, T *Y, int m, hi
       return COSMO_OK;  // the timed launches wrote Y
     }
   }
-  return launch_sell<T, LD>(s, vals, X, Y, m, chunk, st);
+  return launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, chunk, st);
 }
 
 // Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
@@ -687,16 +1036,16 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
   if (f32) {
     const float *x = (const float *)X;
     float *y = (float *)Y;
-    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, x, y, m, st);
-    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, x, y, m, st);
-    return launch_sell_tuned<float, 16>(s, s.v32, x, y, m, st);
+    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, s.pm_v32, x, y, m, st);
+    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, s.pm_v32, x, y, m, st);
+    return launch_sell_tuned<float, 16>(s, s.v32, s.pm_v32, x, y, m, st);
   }
   const double *x = (const double *)X;
   double *y = (double *)Y;
-  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, x, y, m, st);
-  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, x, y, m, st);
-  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, x, y, m, st);
-  return launch_sell_tuned<double, 16>(s, s.v64, x, y, m, st);
+  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, s.pm_v64, x, y, m, st);
+  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, s.pm_v64, x, y, m, st);
+  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, s.pm_v64, x, y, m, st);
+  return launch_sell_tuned<double, 16>(s, s.v64, s.pm_v64, x, y, m, st);
 }
 
 }  // namespace cosmo
