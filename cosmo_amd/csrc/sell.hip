@@ -5,30 +5,41 @@
 // CSBBlock :128-336).  Here the blocks are the basis's own neutron-configuration blocks
 // (consecutive ones merged up to SELL_MAXROWS rows): the rows of one neutron
 // configuration, and as columns the rows of another.  Block (b, t) is non-empty only when
-// the neutron configurations of b and t are one move apart, so a row block meets a
-// hundred or so column tiles, and each entry's X row is one of the tile's rows.
+// the neutron configurations of b and t are one or two moves apart, so a row block meets
+// tens to a hundred or so column tiles, and each entry's X row is one of the tile's rows.
 //
-// Layout (both triangles, as the full symmetric rows the builder makes): per row block b
-// and each of its column tiles t in ascending order, per slice of 64 rows (one wavefront)
-// an ELL panel: slot j of the slice's lane l is entry sl_base[q] + 64 j + l, holding the
-// column inside the tile (u16) and the value (FP32 and FP64 copies); a slice's panel is as
-// long as its longest row in t, shorter rows are padded with zero values pointing at a
-// zero row kept after each staged tile.
-// Row entries keep ascending column order, so every output element is summed over its row
-// in column order: deterministic, the same bits on every call.
+// Entries go to one of three parts, each multiplied in ascending column order within a row
+// (deterministic: the same bits on every call):
+//  * the proton-spectator part (entries between two neutron configurations with the same
+//    proton configuration) on proton-major tiles, k_pm_apply, which writes every row of Y;
+//  * value panels: per row block and each of its column tiles in ascending order, per slice
+//    of 64 rows (one wavefront) an ELL panel: slot j of the slice's lane l is entry
+//    sl_base[q] + 64 j + l, holding the column inside the tile (u16) and the value (FP32
+//    and FP64 copies); a slice's panel is as long as its longest row in t, shorter rows are
+//    padded with zero values pointing at a zero row kept after each staged tile;
+//  * coded panels: the same layout for the (row block, tile) pairs whose entries take at
+//    most SELL_DICT - 1 distinct non-zero values, with a byte per slot that indexes the
+//    pair's dictionary (entry 0 is 0.0) instead of the value.  Off a row block's own tile
+//    the proton-neutron entries of a pair all move the same neutron, so their values are
+//    the interaction's for that move, with signs: a hundred or so values.
+// Lanes take the rows of a block in the order of their numbers of entries (lrow), so the
+// rows of a 64-row slice have similar panel lengths and little padding.
 //
-// Product: one workgroup per row block, one lane per row holding its whole Y row in
-// registers.  Per column tile: the tile's X rows (contiguous in X) go from memory straight
-// into one of two LDS buffers (global_load_lds, no registers involved) one tile ahead, and
-// the first SELL_PF slots of each lane's next panel are loaded into registers, so the
-// loads of the next tile fly while the current one is multiplied out of LDS; one wait and
-// one barrier per tile.
+// Panel product: one workgroup per row block, one lane per row holding its whole Y row in
+// registers.  Per column tile: the tile's X rows (contiguous in X) and, when coded, the
+// pair's dictionary go from memory straight into one of two LDS buffers
+// (global_load_lds, no registers involved) one tile ahead, and the first SELL_PF slots of
+// each lane's next panel are loaded into registers, so the loads of the next tile fly
+// while the current one is multiplied out of LDS; one wait and one barrier per tile.  The
+// coded kernel multiplies the block's value pairs first, one at a time, while its first
+// coded tile lands.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "op.h"
@@ -38,21 +49,40 @@ namespace cosmo {
 #define SELL_MAXROWS 576  // rows of a row block: nine wavefronts, one lane per row
 #define SELL_PF 4         // entry slots per lane loaded one tile ahead
 #define SELL_MIN_N 65536  // smaller operators keep the row kernel
+#define SELL_DICT 256     // entries of a pair's value dictionary (entry 0 is 0.0)
+#define SELL_HASH 1024    // LDS hash slots that collect a pair's distinct values
+#define PM_PF 4           // slots per group in k_pm_apply (the slot arrays carry 4 groups of slack)
 
-struct Sell {
-  int nblk = 0, maxrows = 0, nwaves = 0, maxtiles = 0;
+// one set of panels over the row blocks: per row block a subset of the column tiles it
+// meets (ascending), per (row block, tile, 64-row slice) an ELL panel
+struct Panels {
+  int maxtiles = 0;
   int64_t K = 0, Q = 0, E = 0, Ealloc = 0;
-  int64_t *boff = nullptr;       // [nblk+1] first row of each block
   int32_t *nbr_start = nullptr;  // [nblk+1] into nbr_tile
   int32_t *nbr_tile = nullptr;   // [K] column tiles of each row block, ascending
   int64_t *qstart = nullptr;     // [nblk] first slice of the block: slice (k, w) is qstart + k * nslices + w
   int64_t *sl_base = nullptr;    // [Q+1] first entry slot of each slice (a multiple of 64)
-  uint16_t *cols = nullptr;      // [E] column inside the tile
-  float *v32 = nullptr;          // [E]
-  double *v64 = nullptr;         // [E]
+  uint16_t *cols = nullptr;      // [Ealloc] column inside the tile
+  float *v32 = nullptr;          // value panels: [Ealloc]
+  double *v64 = nullptr;
+  bool coded = false;            // coded panels: a byte per slot into the pair's dictionary
+  uint8_t *code32 = nullptr;     // [Ealloc] codes of the FP32 values
+  uint8_t *code64 = nullptr;     // [Ealloc] codes of the FP64 values
+  float *dict32 = nullptr;       // [dict_total]
+  double *dict64 = nullptr;
+  int32_t *dict_off = nullptr;   // [K] first entry of each pair's dictionary (a multiple of 4)
+  int32_t *dict_n4 = nullptr;    // [K] its length in groups of 4 entries
+  int64_t dict_total = 0;
   uint32_t *pair_c0 = nullptr;   // [K] first X row of each pair's column tile
   int32_t *pair_rows = nullptr;  // [K] its rows
   int4 *items = nullptr;         // [2 nblk] the item of each row block
+};
+
+struct Sell {
+  int nblk = 0, maxrows = 0, nwaves = 0;
+  int64_t *boff = nullptr;       // [nblk+1] first row of each block
+  uint16_t *lrow = nullptr;      // [n] row of block b taken by lane l: boff[b] + lrow[boff[b] + l]
+  Panels P[2];                   // value panels, coded panels
   size_t bytes = 0;
   std::map<int, int> chunk;      // (element size, pitch) -> XCD chunk chosen by timing
   // Proton-spectator part: the entries between two different neutron configurations with
@@ -91,7 +121,16 @@ struct SellView {
   const int32_t *pair_rows;      // and its rows
   const int64_t *sl_base;
   const uint16_t *cols;
+  const int32_t *dict_off;       // coded panels: per pair its dictionary
+  const int32_t *dict_n4;
+  const uint8_t *codes;
+  const uint16_t *lrow;
 };
+
+// the row of the row block starting at r0 that lane (thread) l takes
+__device__ __forceinline__ int64_t lane_row(const uint16_t *__restrict__ lrow, int64_t r0, int l) {
+  return r0 + (lrow ? (int64_t)lrow[r0 + l] : (int64_t)l);
+}
 
 // ---- build -------------------------------------------------------------------------
 // column tiles met by row block b: a bitmap over tiles in LDS from the block's entries
@@ -151,22 +190,24 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
     const int32_t *__restrict__ nbr_start, const int32_t *__restrict__ nbr_tile,
     const int64_t *__restrict__ qstart, int64_t *__restrict__ len, const int64_t *__restrict__ sl_base,
     uint16_t *__restrict__ scol, double *__restrict__ s64, float *__restrict__ s32,
-    const uint64_t *__restrict__ rid) {
+    const uint64_t *__restrict__ rid, const uint16_t *__restrict__ lrow) {
   const int b = blockIdx.x;
   const int64_t r0 = boff[b];
   const int S = (int)(boff[b + 1] - r0), ns = (S + 63) >> 6;
   const int rho = threadIdx.x, lane = rho & 63, w = rho >> 6;
   if (w >= ns) return;  // no barrier below: whole idle wavefronts leave
   const bool on = rho < S;
-  int64_t e = on ? rowptr[r0 + rho] : 0;
-  const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
-  const uint64_t ra = (on && rid) ? rid[r0 + rho] : 0;
+  const int64_t rr = on ? lane_row(lrow, r0, rho) : 0;
+  int64_t e = on ? rowptr[rr] : 0;
+  const int64_t ee = on ? rowptr[rr + 1] : 0;
+  const uint64_t ra = (on && rid) ? rid[rr] : 0;
   const int k0 = nbr_start[b], k1 = nbr_start[b + 1];
   for (int k = k0; k < k1; ++k) {
     const int t = nbr_tile[k];
     const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
     const int64_t q = qstart[b] + (int64_t)(k - k0) * ns + w;
     int c = 0;
+    while (e < ee && cols[e] < cs) ++e;  // entries in tiles of the other panel set
     if (pass == 0) {
       for (; e < ee && cols[e] < ce; ++e)
         if (!pm_entry(rid, ra, cols[e])) ++c;
@@ -240,7 +281,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *_
   const int b = blockIdx.x, rho = threadIdx.x;
   const int64_t r0 = boff[b];
   const bool on = rho < (int)(boff[b + 1] - r0);
-  int64_t e = on ? rowptr[r0 + rho] : 0;
+  int64_t e = on ? rowptr[r0 + rho] : 0;  // lane order does not matter here
   const int64_t ee = on ? rowptr[r0 + rho + 1] : 0;
   const uint64_t ra = (on && rid) ? rid[r0 + rho] : 0;
   if (rho == 0) tot = 0;
@@ -249,6 +290,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *_
     __syncthreads();
     const int t = nbr_tile[k];
     const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
+    while (e < ee && cols[e] < cs) ++e;
     for (; e < ee && cols[e] < ce; ++e)
       if (!pm_entry(rid, ra, cols[e])) atomicOr(&bm[(cols[e] - cs) >> 5], 1u << ((cols[e] - cs) & 31));
     __syncthreads();
@@ -256,6 +298,127 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_distinct(const int64_t *_
     __syncthreads();
   }
   if (rho == 0) atomicAdd(out, (unsigned long long)tot);
+}
+
+// per row: its stored entries outside the proton-spectator part (the lane order's key)
+__global__ void k_row_weight(int64_t n, const int64_t *__restrict__ rowptr, const uint32_t *__restrict__ cols,
+                             const uint64_t *__restrict__ rid, int32_t *__restrict__ wt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int c = 0;
+  const uint64_t ra = rid ? rid[r] : 0;
+  for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e)
+    if (!pm_entry(rid, ra, cols[e])) ++c;
+  wt[r] = c;
+}
+
+// bit patterns of values as hash keys (never all ones: that is a NaN)
+__device__ __forceinline__ unsigned int val_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned long long val_bits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ float bits_val(unsigned int k, float) { return __uint_as_float(k); }
+__device__ __forceinline__ double bits_val(unsigned long long k, double) { return __longlong_as_double((long long)k); }
+__device__ __forceinline__ uint32_t key_hash(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+// The distinct non-zero values of each (row block, tile) pair of a panel set, collected in
+// an LDS hash table (one workgroup per row block, one lane per row, pairs in turn).
+// pass 0: count[k] = their number, or SELL_DICT when there are more than SELL_DICT - 1;
+// pass 1 (pairs of at most SELL_DICT - 1 values): the pair's dictionary at dict_off[k]
+// (entry 0 is 0.0, then the values in hash-table order) and every slot's code, padding
+// slots keeping code 0.
+template <typename T, typename KT>
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_dict(
+    int pass, const int64_t *__restrict__ boff, const int64_t *__restrict__ rowptr,
+    const uint32_t *__restrict__ cols, const T *__restrict__ vals, const uint64_t *__restrict__ rid,
+    const uint16_t *__restrict__ lrow, const int32_t *__restrict__ nbr_start, const int32_t *__restrict__ nbr_tile,
+    int32_t *__restrict__ count, const int64_t *__restrict__ qstart, const int64_t *__restrict__ sl_base,
+    const int32_t *__restrict__ dict_off, T *__restrict__ dict, uint8_t *__restrict__ code) {
+  __shared__ KT tab[SELL_HASH];
+  __shared__ uint8_t tcode[SELL_HASH];
+  __shared__ int nkeys, over;
+  const KT EMPTY = ~(KT)0;
+  const int b = blockIdx.x;
+  const int64_t r0 = boff[b];
+  const int S = (int)(boff[b + 1] - r0), ns = (S + 63) >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool on = tid < S;
+  const int64_t rr = on ? lane_row(lrow, r0, tid) : 0;
+  int64_t e = on ? rowptr[rr] : 0;
+  const int64_t ee = on ? rowptr[rr + 1] : 0;
+  const uint64_t ra = (on && rid) ? rid[rr] : 0;
+  const int k0 = nbr_start[b], k1 = nbr_start[b + 1];
+  for (int k = k0; k < k1; ++k) {
+    for (int i = tid; i < SELL_HASH; i += blockDim.x) tab[i] = EMPTY;
+    if (tid == 0) nkeys = 0, over = 0;
+    __syncthreads();
+    const int t = nbr_tile[k];
+    const uint32_t cs = (uint32_t)boff[t], ce = (uint32_t)boff[t + 1];
+    while (e < ee && cols[e] < cs) ++e;  // entries in tiles of the other panel set
+    const int64_t e_first = e;
+    for (; e < ee && cols[e] < ce; ++e) {
+      if (pm_entry(rid, ra, cols[e])) continue;
+      const KT key = val_bits(vals[e]);
+      if (key == 0) continue;  // +0.0 is entry 0
+      uint32_t h = key_hash(key) & (SELL_HASH - 1);
+      int probe = 0;
+      for (; probe < SELL_HASH; ++probe) {  // bounded: a full table ends the search
+        const KT old = atomicCAS(&tab[h], EMPTY, key);
+        if (old == EMPTY) {
+          atomicAdd(&nkeys, 1);
+          break;
+        }
+        if (old == key) break;
+        h = (h + 1) & (SELL_HASH - 1);
+      }
+      if (probe == SELL_HASH) over = 1;
+    }
+    __syncthreads();
+    if (pass == 0) {
+      if (tid == 0) count[k] = (over || nkeys >= SELL_DICT) ? SELL_DICT : nkeys;
+      __syncthreads();  // the table is cleared for the next pair only after this read
+      continue;
+    }
+    if (w == 0) {  // codes 1, 2, ... in table order
+      const int d0 = dict_off[k];
+      int run = 0;
+      for (int c0 = 0; c0 < SELL_HASH; c0 += 64) {
+        const KT key = tab[c0 + lane];
+        const bool occ = key != EMPTY;
+        const uint64_t mask = __ballot(occ);
+        const int pre = run + __popcll(mask & ((1ull << lane) - 1ull));
+        if (occ) {
+          tcode[c0 + lane] = (uint8_t)(1 + pre);
+          dict[d0 + 1 + pre] = bits_val(key, (T)0);
+        }
+        run += __popcll(mask);
+      }
+      if (lane == 0) dict[d0] = (T)0;
+    }
+    __syncthreads();
+    if (on) {
+      const int64_t base = sl_base[qstart[b] + (int64_t)(k - k0) * ns + w] + lane;
+      int c = 0;
+      for (int64_t x = e_first; x < e; ++x) {
+        if (pm_entry(rid, ra, cols[x])) continue;
+        const KT key = val_bits(vals[x]);
+        uint8_t cd = 0;
+        if (key != 0) {  // present: every key of the pair was inserted above
+          uint32_t h = key_hash(key) & (SELL_HASH - 1);
+          while (tab[h] != key) h = (h + 1) & (SELL_HASH - 1);
+          cd = tcode[h];
+        }
+        code[base + (int64_t)c * 64] = cd;
+        ++c;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ---- product -----------------------------------------------------------------------
@@ -283,11 +446,20 @@ __device__ __forceinline__ int sell_block(int chunk) {
   return ((j / chunk) * 8 + x) * chunk + j % chunk;
 }
 
-template <typename T, int LD>
-__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals,
-                                                            const T *__restrict__ X, T *__restrict__ Y, int m,
-                                                            int chunk, int accum) {
+// LDS bytes of a panel product's pair table and slice offsets (rounded to 16)
+__host__ __device__ inline size_t sell_meta_bytes(int maxtiles, int nwaves) {
+  return (16 * (size_t)maxtiles + 2 * ((size_t)maxtiles * nwaves + 1) + 15) & ~(size_t)15;
+}
+
+// A: the panels multiplied through the pipeline, with vals their slot values (CODED
+// false) or their dictionaries (CODED true); B and bvals (CODED only): the value panels,
+// multiplied first, pair by pair
+template <typename T, int LD, bool CODED>
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals, SellView B,
+                                                            const T *__restrict__ bvals, const T *__restrict__ X,
+                                                            T *__restrict__ Y, int m, int chunk, int accum) {
   typedef typename V16t<T>::type V;
+  typedef typename std::conditional<CODED, uint32_t, T>::type W;  // what a slot holds
   constexpr int VPR = (int)(LD * sizeof(T) / 16);  // 16-byte pieces per X row
   constexpr int VW = (int)(16 / sizeof(T));        // elements per piece
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -301,16 +473,26 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   const bool wave_on = w < ns;
   // LDS: two tile buffers of nthr + 1 rows (row nthr stays zero: the padding slots' row;
   // the direct loads fill rows [0, nthr)), then the block's tile table (first X row,
-  // 16-byte pieces) and its slice offsets (64-slot units from the block's first slot)
+  // 16-byte pieces, dictionary offset and length) and its slice offsets (64-slot units from
+  // the block's first slot), then (coded) two dictionary buffers
   const int zrow = A.nthr;
   V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(zrow + 1) * VPR;
   uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf1 + (size_t)(zrow + 1) * VPR);
   int32_t *mt_np = reinterpret_cast<int32_t *>(mt_c0 + A.maxtiles);
-  uint16_t *mt_off = reinterpret_cast<uint16_t *>(mt_np + A.maxtiles);
+  int32_t *mt_doff = mt_np + A.maxtiles;
+  int32_t *mt_dn = mt_doff + A.maxtiles;
+  uint16_t *mt_off = reinterpret_cast<uint16_t *>(mt_dn + A.maxtiles);
+  T *dict0 = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(mt_c0) +
+                                   sell_meta_bytes(A.maxtiles, A.nthr >> 6));
+  T *dict1 = dict0 + SELL_DICT;
   const int64_t slot0 = A.sl_base[qb];
   for (int i = tid; i < K; i += nthr) {
     mt_c0[i] = A.pair_c0[k0 + i];
     mt_np[i] = A.pair_rows[k0 + i] * VPR;
+    if (CODED) {
+      mt_doff[i] = A.dict_off[k0 + i];
+      mt_dn[i] = A.dict_n4[k0 + i];
+    }
   }
   for (int i = tid; i <= K * ns; i += nthr) mt_off[i] = (uint16_t)((A.sl_base[qb + i] - slot0) >> 6);
   if (tid < 2 * VPR) buf0[(size_t)(tid / VPR) * (zrow + 1) * VPR + (size_t)zrow * VPR + tid % VPR] = V{};
@@ -320,11 +502,12 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   int rot[VPR];
 #pragma unroll
   for (int p = 0; p < VPR; ++p) rot[p] = (p + lane) & (VPR - 1);
+  const int64_t myrow = lane_row(A.lrow, r0, min(tid, S - 1));
   T acc[LD];
 #pragma unroll
   for (int i = 0; i < LD; ++i) acc[i] = (T)0;
   if (accum && tid < S) {  // Y already holds the proton-spectator part of the row
-    const V *yr = reinterpret_cast<const V *>(Y + (r0 + tid) * LD);
+    const V *yr = reinterpret_cast<const V *>(Y + myrow * LD);
 #pragma unroll
     for (int p = 0; p < VPR; ++p) {
       const V u = yr[rot[p]];
@@ -344,7 +527,23 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
                                        (__attribute__((address_space(3))) void *)(dst + (w * 64 + p * nthr)), 16, 0,
                                        0);
   };
-  auto dma_tile = [&](int kk, V *dst) { dma(mt_c0[kk], mt_np[kk], dst); };
+  // a coded pair's dictionary (n4 groups of 4 entries from entry d0) into a dictionary
+  // buffer, by wavefront 0: pieces past its end load its last piece again
+  auto dma_dict = [&](int d0, int n4, T *dd) {
+    if (CODED && w == 0) {
+      const V *src = reinterpret_cast<const V *>(vals + d0);
+      const int np = n4 * 4 / VW;
+#pragma unroll
+      for (int i = 0; i < SELL_DICT / VW / 64; ++i)
+        __builtin_amdgcn_global_load_lds((const void *)(src + min(i * 64 + lane, np - 1)),
+                                         (__attribute__((address_space(3))) void *)(reinterpret_cast<V *>(dd) + i * 64),
+                                         16, 0, 0);
+    }
+  };
+  auto dma_tile = [&](int kk, V *dst, T *dd) {
+    dma(mt_c0[kk], mt_np[kk], dst);
+    if (CODED) dma_dict(mt_doff[kk], mt_dn[kk], dd);
+  };
   // the panel of this wavefront's slice for step kk (the same for the whole wavefront:
   // held in scalar registers); slots [j0, j0 + SELL_PF) of a lane are read unconditionally
   // (the slot arrays carry SELL_PF * 64 slots of slack), so the compiler never makes a
@@ -354,16 +553,17 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     base = slot0 + (int64_t)o * 64 + lane;
     L = __builtin_amdgcn_readfirstlane((int)mt_off[kk * ns + wc + 1]) - o;
   };
-  auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[SELL_PF], T (&h)[SELL_PF]) {
+  auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[SELL_PF], W (&h)[SELL_PF]) {
     (void)L;
 #pragma unroll
     for (int j = 0; j < SELL_PF; ++j) {
       const int64_t p = base + (int64_t)(j0 + j) * 64;
       c[j] = A.cols[p];
-      h[j] = vals[p];
+      if constexpr (CODED) h[j] = A.codes[p];
+      else h[j] = vals[p];
     }
   };
-  auto fma_row = [&](const V *tile, uint32_t c, T h) {
+  auto fma_x = [&](const V *tile, uint32_t c, T h) {
     const V *xr = tile + c * VPR;
 #pragma unroll
     for (int p = 0; p < VPR; ++p) {
@@ -372,6 +572,10 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
       for (int i = 0; i < VW; ++i) acc[p * VW + i] += h * u[i];
     }
   };
+  auto fma_row = [&](const V *tile, const T *dict, uint32_t c, W hw) {
+    if constexpr (CODED) fma_x(tile, c, dict[hw]);
+    else fma_x(tile, c, hw), (void)dict;
+  };
   // One tile step: the next tile's X pieces and the first slot group of its panel are
   // loaded into the "next" register set while the current tile is multiplied; long
   // panels alternate between two groups (one in flight while the other is used).  The
@@ -379,18 +583,18 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   // of a register whose load is in flight makes the compiler wait for every load issued
   // before it, the next tile's included.
   uint32_t c0s[SELL_PF], c1s[SELL_PF], cx[SELL_PF];
-  T h0s[SELL_PF], h1s[SELL_PF], hx[SELL_PF];
+  W h0s[SELL_PF], h1s[SELL_PF], hx[SELL_PF];
   int64_t base0 = 0, base1 = 0;
   int L0 = 0, L1 = 0;
   // slots j0 + j past the panel's end L are skipped here at the use (a branch uniform over
   // the wavefront), not at the load, so the loads stay in flight until the group is reached
-  auto group = [&](const V *cur, const uint32_t (&c)[SELL_PF], const T (&h)[SELL_PF], int j0, int L) {
+  auto group = [&](const V *cur, const T *cd, const uint32_t (&c)[SELL_PF], const W (&h)[SELL_PF], int j0, int L) {
 #pragma unroll
     for (int j = 0; j < SELL_PF; ++j)
-      if (j0 + j < L) fma_row(cur, c[j], h[j]);
+      if (j0 + j < L) fma_row(cur, cd, c[j], h[j]);
   };
-  auto step = [&](int kk, const V *cur, V *nxt, uint32_t (&ca)[SELL_PF], T (&ha)[SELL_PF], int64_t base_a, int La,
-                  uint32_t (&cb)[SELL_PF], T (&hb)[SELL_PF], int64_t &base_b, int &Lb) {
+  auto step = [&](int kk, const V *cur, const T *cd, V *nxt, T *nd, uint32_t (&ca)[SELL_PF], W (&ha)[SELL_PF],
+                  int64_t base_a, int La, uint32_t (&cb)[SELL_PF], W (&hb)[SELL_PF], int64_t &base_b, int &Lb) {
     // the next step's loads are issued unconditionally (the last step reloads its own tile
     // and slots, unused): a load under a branch would make the first use after the join
     // wait for every load in flight, the next tile's included
@@ -398,40 +602,84 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     const int kn = more ? kk + 1 : kk;
     slice_of(kn, base_b, Lb);
     load_slots(base_b, Lb, 0, cb, hb);
-    if (more) dma_tile(kn, nxt);  // the buffer read in the previous step: free since its barrier
+    if (more) dma_tile(kn, nxt, nd);  // the buffers read in the previous step: free since its barrier
     if (wave_on && La > 0) {
-      group(cur, ca, ha, 0, La);
+      group(cur, cd, ca, ha, 0, La);
       if (SELL_PF < La) {
         load_slots(base_a, La, SELL_PF, cx, hx);
         for (int j0 = SELL_PF; j0 < La; j0 += 2 * SELL_PF) {
           if (j0 + SELL_PF < La) load_slots(base_a, La, j0 + SELL_PF, ca, ha);
-          group(cur, cx, hx, j0, La);
+          group(cur, cd, cx, hx, j0, La);
           if (j0 + SELL_PF >= La) break;
           if (j0 + 2 * SELL_PF < La) load_slots(base_a, La, j0 + 2 * SELL_PF, cx, hx);
-          group(cur, ca, ha, j0 + SELL_PF, La);
+          group(cur, cd, ca, ha, j0 + SELL_PF, La);
         }
       }
     }
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wavefront's direct loads have landed
     __syncthreads();
   };
+  // the first pipelined tile (and its dictionary) straight from memory, beside the table's
+  // loads; it lands while the value pairs (coded kernel) are multiplied in the other buffer
   if (K > 0) {
-    // the first tile and slots straight from memory, issued beside the table's loads; the
-    // barrier after them also publishes the table
+    dma(A.pair_c0[k0], A.pair_rows[k0] * VPR, buf0);
+    if (CODED) dma_dict(A.dict_off[k0], A.dict_n4[k0], dict0);
+  }
+  if constexpr (CODED) {
+    // value pairs one at a time in buf1: tile, wait, multiply; slot groups of SELL_PF with
+    // the next group in flight
+    const int4 e0 = B.items[2 * b], e1 = B.items[2 * b + 1];
+    const int bk0 = e0.x, bK = e0.y;
+    uint32_t cva[SELL_PF], cvb[SELL_PF];
+    T hva[SELL_PF], hvb[SELL_PF];
+    auto vload = [&](int64_t base, int j0, uint32_t (&c)[SELL_PF], T (&h)[SELL_PF]) {
+#pragma unroll
+      for (int j = 0; j < SELL_PF; ++j) {
+        const int64_t p = base + (int64_t)(j0 + j) * 64;
+        c[j] = B.cols[p];
+        h[j] = bvals[p];
+      }
+    };
+    auto vgroup = [&](const uint32_t (&c)[SELL_PF], const T (&h)[SELL_PF], int j0, int L) {
+#pragma unroll
+      for (int j = 0; j < SELL_PF; ++j)
+        if (j0 + j < L) fma_x(buf1, c[j], h[j]);
+    };
+    for (int i = 0; i < bK; ++i) {
+      const int64_t q = (int64_t)e1.y + (int64_t)i * ns + wc;
+      const int64_t sb = B.sl_base[q];
+      const int L = __builtin_amdgcn_readfirstlane((int)((B.sl_base[q + 1] - sb) >> 6));
+      const int64_t base = sb + lane;
+      dma(B.pair_c0[bk0 + i], B.pair_rows[bk0 + i] * VPR, buf1);
+      vload(base, 0, cva, hva);
+      __builtin_amdgcn_s_waitcnt(0x0f70);
+      __syncthreads();
+      if (wave_on)
+        for (int j0 = 0; j0 < L; j0 += 2 * SELL_PF) {
+          if (j0 + SELL_PF < L) vload(base, j0 + SELL_PF, cvb, hvb);
+          vgroup(cva, hva, j0, L);
+          if (j0 + SELL_PF >= L) break;
+          if (j0 + 2 * SELL_PF < L) vload(base, j0 + 2 * SELL_PF, cva, hva);
+          vgroup(cvb, hvb, j0 + SELL_PF, L);
+        }
+      __syncthreads();  // buf1 is loaded again
+    }
+  }
+  if (K > 0) {
+    // the first slots; the barrier after them also publishes the table
     const int64_t q0 = A.sl_base[qb + wc];
     base0 = q0 + lane;
     L0 = __builtin_amdgcn_readfirstlane((int)((A.sl_base[qb + wc + 1] - q0) >> 6));
     load_slots(base0, L0, 0, c0s, h0s);
-    dma(A.pair_c0[k0], A.pair_rows[k0] * VPR, buf0);
     __builtin_amdgcn_s_waitcnt(0x0f70);
     __syncthreads();
     for (int kk = 0; kk < K; kk += 2) {
-      step(kk, buf0, buf1, c0s, h0s, base0, L0, c1s, h1s, base1, L1);
-      if (kk + 1 < K) step(kk + 1, buf1, buf0, c1s, h1s, base1, L1, c0s, h0s, base0, L0);
+      step(kk, buf0, dict0, buf1, dict1, c0s, h0s, base0, L0, c1s, h1s, base1, L1);
+      if (kk + 1 < K) step(kk + 1, buf1, dict1, buf0, dict0, c1s, h1s, base1, L1, c0s, h0s, base0, L0);
     }
   }
   if (tid < S) {
-    T *y = Y + (r0 + tid) * LD;
+    T *y = Y + myrow * LD;
 #pragma unroll
     for (int p = 0; p < VPR; ++p) {
       // accumulator block p holds piece rot[p] of the row
@@ -559,11 +807,14 @@ static std::mutex g_sell_mu;
 void free_sell(Op &op) {
   Sell *s = (Sell *)op.sell;
   if (!s) return;
-  for (void *p : {(void *)s->boff, (void *)s->nbr_start, (void *)s->nbr_tile, (void *)s->qstart,
-                  (void *)s->sl_base, (void *)s->cols, (void *)s->v32, (void *)s->v64, (void *)s->pair_c0,
-                  (void *)s->pair_rows, (void *)s->items, (void *)s->rid, (void *)s->cls_local,
-                  (void *)s->pm_rows, (void *)s->pm_items, (void *)s->pm_base, (void *)s->pm_cols,
-                  (void *)s->pm_v32, (void *)s->pm_v64})
+  for (Panels &P : s->P)
+    for (void *p : {(void *)P.nbr_start, (void *)P.nbr_tile, (void *)P.qstart, (void *)P.sl_base, (void *)P.cols,
+                    (void *)P.v32, (void *)P.v64, (void *)P.code32, (void *)P.code64, (void *)P.dict32,
+                    (void *)P.dict64, (void *)P.dict_off, (void *)P.dict_n4, (void *)P.pair_c0,
+                    (void *)P.pair_rows, (void *)P.items})
+      dfree(p);
+  for (void *p : {(void *)s->boff, (void *)s->lrow, (void *)s->rid, (void *)s->cls_local, (void *)s->pm_rows,
+                  (void *)s->pm_items, (void *)s->pm_base, (void *)s->pm_cols, (void *)s->pm_v32, (void *)s->pm_v64})
     dfree(p);
   delete s;
   op.sell = nullptr;
@@ -598,12 +849,6 @@ static std::vector<int64_t> sell_partition(const Op &op) {
   return out;
 }
 
-// LDS bytes of the per-block tile table and slice offsets
-static size_t sell_meta_bytes(const Sell &s) {
-  return 8 * (size_t)s.maxtiles + 2 * ((size_t)s.maxtiles * s.nwaves + 1) + 16;
-}
-
-#define PM_PF 4  // slots per group in k_pm_apply (the slot arrays carry 4 groups of slack)
 
 // LDS bytes of k_pm_apply at a row pitch of `row` bytes: zero row, tile, member rows
 static size_t pm_lds_bytes(const Sell &s, size_t row) {
@@ -714,9 +959,116 @@ static int build_pm(const Op &op, Sell &s, hipStream_t st) {
   return COSMO_OK;
 }
 
+// one panel set from its per-block tile lists (hs: first pair of each block, ht: tiles):
+// panel lengths, slot offsets and columns, the pair table and the block items
+static int build_panels(Sell &s, Panels &P, const std::vector<int64_t> &hb, const std::vector<int32_t> &hs,
+                        const std::vector<int32_t> &ht, const CsrDev &M, const uint64_t *rid, hipStream_t st) {
+  const int nblk = s.nblk;
+  P.K = hs[nblk];
+  std::vector<int64_t> hq(nblk);
+  int64_t Q = 0;
+  for (int b = 0; b < nblk; ++b) {
+    hq[b] = Q;
+    Q += (int64_t)(hs[b + 1] - hs[b]) * ((hb[b + 1] - hb[b] + 63) / 64);
+  }
+  if (Q >= INT32_MAX) return fail(COSMO_EUNSUP, "tiled layout: too many slices");
+  P.Q = Q;
+  int64_t *len = nullptr;
+  if (dmalloc(&P.nbr_start, sizeof(int32_t) * (nblk + 1)) ||
+      dmalloc(&P.nbr_tile, sizeof(int32_t) * std::max<int64_t>(P.K, 1)) ||
+      dmalloc(&P.qstart, sizeof(int64_t) * nblk) || dmalloc(&P.sl_base, sizeof(int64_t) * (Q + 1)) ||
+      dmalloc(&len, sizeof(int64_t) * std::max<int64_t>(Q, 1)))
+    return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+  HIPCHK(hipMemcpyAsync(P.nbr_start, hs.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice, st));
+  if (P.K) HIPCHK(hipMemcpyAsync(P.nbr_tile, ht.data(), sizeof(int32_t) * P.K, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.qstart, hq.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, st));
+  const int thr = s.nwaves * 64;
+  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 0, (const int64_t *)s.boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, (const int32_t *)P.nbr_start, (const int32_t *)P.nbr_tile,
+                     (const int64_t *)P.qstart, len, (const int64_t *)nullptr, (uint16_t *)nullptr,
+                     (double *)nullptr, (float *)nullptr, rid, (const uint16_t *)s.lrow);
+  HIPCHK(hipGetLastError());
+  std::vector<int64_t> hl(std::max<int64_t>(Q, 1)), hbase(Q + 1, 0);
+  if (Q) HIPCHK(hipMemcpyAsync(hl.data(), len, sizeof(int64_t) * Q, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  dfree(len);
+  for (int64_t q = 0; q < Q; ++q) hbase[q + 1] = hbase[q] + 64 * hl[q];
+  P.E = hbase[Q];
+  // the product keeps each block's slice offsets in LDS as 16-bit counts of 64 slots
+  for (int b = 0; b < nblk; ++b) {
+    P.maxtiles = std::max(P.maxtiles, hs[b + 1] - hs[b]);
+    const int64_t qe = hq[b] + (int64_t)(hs[b + 1] - hs[b]) * ((hb[b + 1] - hb[b] + 63) / 64);
+    if ((hbase[qe] - hbase[hq[b]]) / 64 > 65535) return fail(COSMO_EUNSUP, "a row block too long for the tiled product");
+  }
+  P.Ealloc = P.E + 64 * SELL_PF;  // a lane reads up to SELL_PF slots past its slice's panel
+  if (dmalloc(&P.cols, sizeof(uint16_t) * P.Ealloc)) return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+  HIPCHK(hipMemcpyAsync(P.sl_base, hbase.data(), sizeof(int64_t) * (Q + 1), hipMemcpyHostToDevice, st));
+  // padding slots point at the zero row kept after every staged tile (index nwaves * 64)
+  HIPCHK(hipMemsetD16Async((hipDeviceptr_t)P.cols, (unsigned short)(s.nwaves * 64), P.Ealloc, st));
+  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 1, (const int64_t *)s.boff,
+                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
+                     (const float *)nullptr, (const int32_t *)P.nbr_start, (const int32_t *)P.nbr_tile,
+                     (const int64_t *)P.qstart, (int64_t *)nullptr, (const int64_t *)P.sl_base, P.cols,
+                     (double *)nullptr, (float *)nullptr, rid, (const uint16_t *)s.lrow);
+  HIPCHK(hipGetLastError());
+  // the product's per-pair tile table and whole-block items (rows and slices held as int32)
+  std::vector<uint32_t> pc(std::max<int64_t>(P.K, 1), 0);
+  std::vector<int32_t> pr(std::max<int64_t>(P.K, 1), 0);
+  for (int64_t k = 0; k < P.K; ++k) {
+    const int t = ht[k];
+    pc[k] = (uint32_t)hb[t];
+    pr[k] = (int32_t)(hb[t + 1] - hb[t]);
+  }
+  std::vector<int4> it(2 * (size_t)nblk);
+  for (int b = 0; b < nblk; ++b) {
+    it[2 * b] = make_int4(hs[b], hs[b + 1] - hs[b], (int)(hb[b + 1] - hb[b]), 0);
+    it[2 * b + 1] = make_int4((int)hb[b], (int)hq[b], 0, 0);
+  }
+  if (dmalloc(&P.pair_c0, sizeof(uint32_t) * pc.size()) || dmalloc(&P.pair_rows, sizeof(int32_t) * pr.size()) ||
+      dmalloc(&P.items, sizeof(int4) * it.size()))
+    return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
+  HIPCHK(hipMemcpyAsync(P.pair_c0, pc.data(), sizeof(uint32_t) * pc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.pair_rows, pr.data(), sizeof(int32_t) * pr.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P.items, it.data(), sizeof(int4) * it.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s.bytes += sizeof(int32_t) * (nblk + 1 + P.K) + sizeof(int64_t) * (nblk + Q + 1) + (size_t)P.Ealloc * 2 +
+             8 * (size_t)P.K + 32 * (size_t)nblk;
+  return COSMO_OK;
+}
+
+// layout statistics of one panel set on stderr
+static void panel_stats(const char *what, const Sell &s, const Panels &P, const std::vector<int64_t> &hb,
+                        const std::vector<int32_t> &ht, const CsrDev &M, const uint64_t *rid, hipStream_t st) {
+  const int nblk = s.nblk;
+  const int64_t n = hb[nblk];
+  int64_t staged = 0;  // every (row block, column tile) pair loads the tile's rows
+  for (int64_t k = 0; k < P.K; ++k) staged += hb[ht[k] + 1] - hb[ht[k]];
+  unsigned long long *dd = nullptr, distinct = 0;
+  if (P.K && !dmalloc(&dd, sizeof(unsigned long long))) {
+    if (!hipMemsetAsync(dd, 0, sizeof(unsigned long long), st)) {
+      hipLaunchKernelGGL(k_sell_distinct, dim3(nblk), dim3(s.nwaves * 64), 0, st, (const int64_t *)s.boff,
+                         (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)P.nbr_start,
+                         (const int32_t *)P.nbr_tile, dd, rid);
+      if (hipMemcpyAsync(&distinct, dd, sizeof(distinct), hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st))
+        distinct = 0;
+    }
+    dfree(dd);
+  }
+  fprintf(stderr,
+          "[sell] %s panels: pairs=%lld (<= %d per block) slices=%lld slots=%lld staged X rows=%lld (%.3f n) "
+          "referenced=%llu (%.3f n)\n",
+          what, (long long)P.K, P.maxtiles, (long long)P.Q, (long long)P.E, (long long)staged,
+          (double)staged / (double)n, distinct, (double)distinct / (double)n);
+  if (P.coded)
+    fprintf(stderr, "[sell] %s panels: dictionary entries %lld (%.1f per pair)\n", what, (long long)P.dict_total,
+            P.K ? (double)P.dict_total / (double)P.K : 0.0);
+}
+
 static int build_sell(Op &op, hipStream_t st) {
   const CsrDev &M = op.csr;
   const int64_t n = op.n;
+  if (n >= INT32_MAX) return fail(COSMO_EUNSUP, "tiled layout: too many rows");
   std::vector<int64_t> hb = sell_partition(op);
   const int nblk = (int)hb.size() - 1;
   if (nblk <= 0) return fail(COSMO_EINVAL, "empty operator");
@@ -727,7 +1079,9 @@ static int build_sell(Op &op, hipStream_t st) {
   s->nblk = nblk;
   s->maxrows = maxrows;
   s->nwaves = (maxrows + 63) / 64;
+  int32_t *blk_of = nullptr, *cnt = nullptr, *all_start = nullptr, *all_tile = nullptr, *dcnt = nullptr;
   auto bail = [&](int rc) {
+    for (void *p : {(void *)blk_of, (void *)cnt, (void *)all_start, (void *)all_tile, (void *)dcnt}) dfree(p);
     op.sell = s;
     free_sell(op);
     return rc;
@@ -739,13 +1093,43 @@ static int build_sell(Op &op, hipStream_t st) {
     if (s->split && (rc = build_pm(op, *s, st))) return bail(rc);
   }
   const uint64_t *rid = s->split ? s->rid : nullptr;
+  // lane order: the rows of each block by their numbers of entries outside the
+  // proton-spectator part, most first (ties in row order)
+  {
+    int32_t *wt = nullptr;
+    if (dmalloc(&wt, sizeof(int32_t) * n) || dmalloc(&s->lrow, sizeof(uint16_t) * n)) {
+      dfree(wt);
+      return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
+    }
+    hipLaunchKernelGGL(k_row_weight, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                       (const int64_t *)M.rowptr, (const uint32_t *)M.cols, rid, wt);
+    std::vector<int32_t> hw(n);
+    if (hipGetLastError() || hipMemcpyAsync(hw.data(), wt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st)) {
+      dfree(wt);
+      return bail(fail(COSMO_EHIP, "tiled layout: row weights failed"));
+    }
+    dfree(wt);
+    std::vector<uint16_t> hl(n);
+    std::vector<int> idx;
+    for (int b = 0; b < nblk; ++b) {
+      const int S = (int)(hb[b + 1] - hb[b]);
+      idx.resize(S);
+      for (int i = 0; i < S; ++i) idx[i] = i;
+      const int32_t *wb = hw.data() + hb[b];
+      std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return wb[a] > wb[c]; });
+      for (int i = 0; i < S; ++i) hl[hb[b] + i] = (uint16_t)idx[i];
+    }
+    HIPCHK(hipMemcpyAsync(s->lrow, hl.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s->bytes += sizeof(uint16_t) * n;
+  }
+  // every (row block, column tile) pair with entries outside the proton-spectator part
   std::vector<int32_t> hblk(n);
   for (int b = 0; b < nblk; ++b)
     for (int64_t r = hb[b]; r < hb[b + 1]; ++r) hblk[r] = b;
-  int32_t *blk_of = nullptr, *cnt = nullptr;
-  int64_t *len = nullptr;
   if (dmalloc(&s->boff, sizeof(int64_t) * (nblk + 1)) || dmalloc(&blk_of, sizeof(int32_t) * n) ||
-      dmalloc(&cnt, sizeof(int32_t) * nblk) || dmalloc(&s->nbr_start, sizeof(int32_t) * (nblk + 1)))
+      dmalloc(&cnt, sizeof(int32_t) * nblk) || dmalloc(&all_start, sizeof(int32_t) * (nblk + 1)))
     return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
   HIPCHK(hipMemcpyAsync(s->boff, hb.data(), sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(blk_of, hblk.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
@@ -758,125 +1142,68 @@ static int build_sell(Op &op, hipStream_t st) {
   HIPCHK(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int b = 0; b < nblk; ++b) hs[b + 1] = hs[b] + hc[b];
-  s->K = hs[nblk];
-  std::vector<int64_t> hq(nblk);
-  int64_t Q = 0;
-  for (int b = 0; b < nblk; ++b) {
-    hq[b] = Q;
-    Q += (int64_t)hc[b] * ((hb[b + 1] - hb[b] + 63) / 64);
-  }
-  s->Q = Q;
-  if (dmalloc(&s->nbr_tile, sizeof(int32_t) * std::max<int64_t>(s->K, 1)) ||
-      dmalloc(&s->qstart, sizeof(int64_t) * nblk) || dmalloc(&len, sizeof(int64_t) * std::max<int64_t>(Q, 1)) ||
-      dmalloc(&s->sl_base, sizeof(int64_t) * (Q + 1)))
+  const int64_t K = hs[nblk];
+  if (dmalloc(&all_tile, sizeof(int32_t) * std::max<int64_t>(K, 1)) ||
+      dmalloc(&dcnt, sizeof(int32_t) * std::max<int64_t>(K, 1)))
     return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
-  HIPCHK(hipMemcpyAsync(s->nbr_start, hs.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(s->qstart, hq.data(), sizeof(int64_t) * nblk, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(all_start, hs.data(), sizeof(int32_t) * (nblk + 1), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_sell_tiles, dim3(nblk), dim3(256), bm_bytes, st, 1, (const int64_t *)s->boff,
                      (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)blk_of, nblk, cnt,
-                     (const int32_t *)s->nbr_start, s->nbr_tile, rid);
+                     (const int32_t *)all_start, all_tile, rid);
   HIPCHK(hipGetLastError());
-  const int thr = s->nwaves * 64;
-  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 0, (const int64_t *)s->boff,
-                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
-                     (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
-                     (const int64_t *)s->qstart, len, (const int64_t *)nullptr, (uint16_t *)nullptr,
-                     (double *)nullptr, (float *)nullptr, rid);
+  // the distinct FP64 values of each pair (an FP32 copy can only have fewer)
+  hipLaunchKernelGGL((k_sell_dict<double, unsigned long long>), dim3(nblk), dim3(s->nwaves * 64), 0, st, 0,
+                     (const int64_t *)s->boff, (const int64_t *)M.rowptr, (const uint32_t *)M.cols,
+                     (const double *)M.v64, rid, (const uint16_t *)s->lrow, (const int32_t *)all_start,
+                     (const int32_t *)all_tile, dcnt, (const int64_t *)nullptr, (const int64_t *)nullptr,
+                     (const int32_t *)nullptr, (double *)nullptr, (uint8_t *)nullptr);
   HIPCHK(hipGetLastError());
-  std::vector<int64_t> hl(Q), hbase(Q + 1, 0);
-  HIPCHK(hipMemcpyAsync(hl.data(), len, sizeof(int64_t) * Q, hipMemcpyDeviceToHost, st));
+  std::vector<int32_t> ht(std::max<int64_t>(K, 1)), hd(std::max<int64_t>(K, 1), SELL_DICT);
+  if (K) {
+    HIPCHK(hipMemcpyAsync(ht.data(), all_tile, sizeof(int32_t) * K, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hd.data(), dcnt, sizeof(int32_t) * K, hipMemcpyDeviceToHost, st));
+  }
   HIPCHK(hipStreamSynchronize(st));
-  for (int64_t q = 0; q < Q; ++q) hbase[q + 1] = hbase[q] + 64 * hl[q];
-  s->E = hbase[Q];
-  // the product keeps each block's slice offsets in LDS as 16-bit counts of 64 slots
+  for (void *p : {(void *)blk_of, (void *)cnt, (void *)all_start, (void *)all_tile, (void *)dcnt}) dfree(p);
+  blk_of = cnt = all_start = all_tile = dcnt = nullptr;
+  // pairs of at most SELL_DICT - 1 distinct non-zero values go to the coded panels
+  std::vector<int32_t> hs0(nblk + 1, 0), hs1(nblk + 1, 0), ht0, ht1, hn1;
   for (int b = 0; b < nblk; ++b) {
-    s->maxtiles = std::max(s->maxtiles, hc[b]);
-    const int64_t qe = hq[b] + (int64_t)hc[b] * ((hb[b + 1] - hb[b] + 63) / 64);
-    if ((hbase[qe] - hbase[hq[b]]) / 64 > 65535)
-      return bail(fail(COSMO_EUNSUP, "a row block too long for the tiled product"));
+    for (int k = hs[b]; k < hs[b + 1]; ++k) {
+      if (hd[k] < SELL_DICT) ht1.push_back(ht[k]), hn1.push_back(hd[k]);
+      else ht0.push_back(ht[k]);
+    }
+    hs0[b + 1] = (int32_t)ht0.size();
+    hs1[b + 1] = (int32_t)ht1.size();
   }
-
-  // slack: a lane reads up to SELL_PF slots past its slice's panel
-  const int64_t E = s->E + 64 * SELL_PF;
-  if (dmalloc(&s->cols, sizeof(uint16_t) * E)) return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
-  s->Ealloc = E;
-  HIPCHK(hipMemcpyAsync(s->sl_base, hbase.data(), sizeof(int64_t) * (Q + 1), hipMemcpyHostToDevice, st));
-  // padding slots point at the zero row kept after every staged tile (index nwaves * 64)
-  HIPCHK(hipMemsetD16Async((hipDeviceptr_t)s->cols, (unsigned short)(s->nwaves * 64), E, st));
-  hipLaunchKernelGGL(k_sell_slots, dim3(nblk), dim3(thr), 0, st, 1, (const int64_t *)s->boff,
-                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)nullptr,
-                     (const float *)nullptr, (const int32_t *)s->nbr_start, (const int32_t *)s->nbr_tile,
-                     (const int64_t *)s->qstart, (int64_t *)nullptr, (const int64_t *)s->sl_base, s->cols,
-                     (double *)nullptr, (float *)nullptr, rid);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
-  dfree(blk_of);
-  dfree(cnt);
-  dfree(len);
-  // the column tile of each pair, on the host for the item table (and the statistics)
-  std::vector<int32_t> ht(std::max<int64_t>(s->K, 1));
-  HIPCHK(hipMemcpy(ht.data(), s->nbr_tile, sizeof(int32_t) * s->K, hipMemcpyDeviceToHost));
-  // the product's per-pair tile table and whole-block items (rows and slices held as int32)
-  if (Q >= INT32_MAX || n >= INT32_MAX) return bail(fail(COSMO_EUNSUP, "tiled layout: too many slices"));
-  {
-    std::vector<uint32_t> pc(std::max<int64_t>(s->K, 1), 0);
-    std::vector<int32_t> pr(std::max<int64_t>(s->K, 1), 0);
-    for (int64_t k = 0; k < s->K; ++k) {
-      const int t = ht[k];
-      pc[k] = (uint32_t)hb[t];
-      pr[k] = (int32_t)(hb[t + 1] - hb[t]);
+  int rc = build_panels(*s, s->P[0], hb, hs0, ht0, M, rid, st);
+  if (!rc && !ht1.empty()) rc = build_panels(*s, s->P[1], hb, hs1, ht1, M, rid, st);
+  if (rc) return bail(rc);
+  Panels &C = s->P[1];
+  if (C.K) {
+    C.coded = true;
+    std::vector<int32_t> off(C.K), n4(C.K);
+    int64_t tot = 0;
+    for (int64_t k = 0; k < C.K; ++k) {
+      off[k] = (int32_t)tot;
+      n4[k] = (hn1[k] + 1 + 3) / 4;  // entry 0 (0.0) and the values, in groups of 4
+      tot += 4 * (int64_t)n4[k];
     }
-    std::vector<int4> it(2 * (size_t)nblk);
-    for (int b = 0; b < nblk; ++b) {
-      it[2 * b] = make_int4(hs[b], hs[b + 1] - hs[b], (int)(hb[b + 1] - hb[b]), 0);
-      it[2 * b + 1] = make_int4((int)hb[b], (int)hq[b], 0, 0);
-    }
-    if (dmalloc(&s->pair_c0, sizeof(uint32_t) * pc.size()) || dmalloc(&s->pair_rows, sizeof(int32_t) * pr.size()) ||
-        dmalloc(&s->items, sizeof(int4) * it.size()))
+    if (tot >= INT32_MAX) return bail(fail(COSMO_EUNSUP, "tiled layout: dictionaries too large"));
+    C.dict_total = tot;
+    if (dmalloc(&C.dict_off, sizeof(int32_t) * C.K) || dmalloc(&C.dict_n4, sizeof(int32_t) * C.K))
       return bail(fail(COSMO_ENOMEM, "tiled layout: allocation failed"));
-    HIPCHK(hipMemcpyAsync(s->pair_c0, pc.data(), sizeof(uint32_t) * pc.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->pair_rows, pr.data(), sizeof(int32_t) * pr.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s->items, it.data(), sizeof(int4) * it.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(C.dict_off, off.data(), sizeof(int32_t) * C.K, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(C.dict_n4, n4.data(), sizeof(int32_t) * C.K, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
+    s->bytes += 8 * (size_t)C.K;
   }
-  s->bytes += sizeof(int64_t) * (nblk + 1) * 2 + sizeof(int32_t) * (nblk + 1 + s->K) + sizeof(int64_t) * (Q + 1) +
-             (size_t)E * 2 + 8 * (size_t)s->K + 32 * (size_t)nblk;
+  s->bytes += sizeof(int64_t) * (nblk + 1);
   if (g_sell_stats) {
-    // X rows one product stages: every (row block, column tile) pair loads the tile's rows
-    int64_t staged = 0;
-    for (int64_t t = 0; t < s->K; ++t) staged += hb[ht[t] + 1] - hb[ht[t]];
-    // the same if g neighbouring row blocks staged the union of their tiles once
-    for (int g : {2, 4, 8}) {
-      int64_t sg = 0;
-      std::vector<char> seen(nblk, 0);
-      std::vector<int32_t> list;
-      for (int b0 = 0; b0 < nblk; b0 += g) {
-        list.clear();
-        for (int b = b0; b < std::min(nblk, b0 + g); ++b)
-          for (int k = hs[b]; k < hs[b + 1]; ++k)
-            if (!seen[ht[k]]) seen[ht[k]] = 1, list.push_back(ht[k]);
-        for (int32_t t : list) sg += hb[t + 1] - hb[t], seen[t] = 0;
-      }
-      fprintf(stderr, "[sell] groups of %d blocks: staged X rows %lld (%.3f n)\n", g, (long long)sg,
-              (double)sg / (double)n);
-    }
-    unsigned long long *dd = nullptr, distinct = 0;
-    if (!dmalloc(&dd, sizeof(unsigned long long))) {
-      HIPCHK(hipMemsetAsync(dd, 0, sizeof(unsigned long long), st));
-      hipLaunchKernelGGL(k_sell_distinct, dim3(nblk), dim3(thr), 0, st, (const int64_t *)s->boff,
-                         (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const int32_t *)s->nbr_start,
-                         (const int32_t *)s->nbr_tile, dd, rid);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(&distinct, dd, sizeof(distinct), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      dfree(dd);
-    }
-    fprintf(stderr,
-            "[sell] n=%lld blocks=%d (rows <= %d) tiles=%lld slices=%lld slots=%lld stored=%lld "
-            "padding=%.4f staged X rows=%lld (%.3f n) referenced=%llu (%.3f n)\n",
-            (long long)n, nblk, maxrows, (long long)s->K, (long long)Q, (long long)s->E, (long long)M.nnz,
-            M.nnz ? (double)s->E / (double)M.nnz : 0.0, (long long)staged, (double)staged / (double)n, distinct,
-            (double)distinct / (double)n);
+    fprintf(stderr, "[sell] n=%lld blocks=%d (rows <= %d) stored=%lld\n", (long long)n, nblk, maxrows,
+            (long long)M.nnz);
+    panel_stats("value", *s, s->P[0], hb, ht0, M, rid, st);
+    if (C.K) panel_stats("coded", *s, C, hb, ht1, M, rid, st);
     if (s->split)
       fprintf(stderr, "[sell] proton-spectator part: units=%d (class rows <= %d) slices=%lld slots=%lld\n",
               s->pm_units, s->pm_maxC, (long long)s->pm_Q, (long long)s->pm_E);
@@ -886,29 +1213,59 @@ static int build_sell(Op &op, hipStream_t st) {
 }
 
 // the slot values of one precision, filled on its first product (the FP32 and the FP64
-// copies are only kept for the precisions in use)
+// copies are only kept for the precisions in use): values of the value panels and of the
+// proton-spectator part, codes and dictionaries of the coded panels
 static int sell_values(Op &op, bool f32, hipStream_t st) {
   Sell &s = *(Sell *)op.sell;
-  if (f32 ? s.v32 != nullptr : s.v64 != nullptr) return COSMO_OK;
+  Panels &V = s.P[0], &C = s.P[1];
+  if (f32 ? V.v32 != nullptr : V.v64 != nullptr) return COSMO_OK;
   const CsrDev &M = op.csr;
+  const uint64_t *rid = s.split ? (const uint64_t *)s.rid : (const uint64_t *)nullptr;
+  const size_t es = f32 ? sizeof(float) : sizeof(double);
   void *dst = nullptr;
-  if (dmalloc_void(&dst, (f32 ? sizeof(float) : sizeof(double)) * s.Ealloc, st))
-    return fail(COSMO_ENOMEM, "tiled layout: value allocation failed");
-  HIPCHK(hipMemsetAsync(dst, 0, (f32 ? sizeof(float) : sizeof(double)) * s.Ealloc, st));
-  hipLaunchKernelGGL(k_sell_slots, dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1, (const int64_t *)s.boff,
-                     (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)M.v64,
-                     (const float *)M.v32, (const int32_t *)s.nbr_start, (const int32_t *)s.nbr_tile,
-                     (const int64_t *)s.qstart, (int64_t *)nullptr, (const int64_t *)s.sl_base, (uint16_t *)nullptr,
-                     f32 ? (double *)nullptr : (double *)dst, f32 ? (float *)dst : (float *)nullptr,
-                     s.split ? (const uint64_t *)s.rid : (const uint64_t *)nullptr);
+  if (dmalloc_void(&dst, es * V.Ealloc, st)) return fail(COSMO_ENOMEM, "tiled layout: value allocation failed");
+  HIPCHK(hipMemsetAsync(dst, 0, es * V.Ealloc, st));
+  if (V.K)
+    hipLaunchKernelGGL(k_sell_slots, dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1, (const int64_t *)s.boff,
+                       (const int64_t *)M.rowptr, (const uint32_t *)M.cols, (const double *)M.v64,
+                       (const float *)M.v32, (const int32_t *)V.nbr_start, (const int32_t *)V.nbr_tile,
+                       (const int64_t *)V.qstart, (int64_t *)nullptr, (const int64_t *)V.sl_base, (uint16_t *)nullptr,
+                       f32 ? (double *)nullptr : (double *)dst, f32 ? (float *)dst : (float *)nullptr, rid,
+                       (const uint16_t *)s.lrow);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
-  if (f32) s.v32 = (float *)dst;
-  else s.v64 = (double *)dst;
-  s.bytes += (f32 ? sizeof(float) : sizeof(double)) * (size_t)s.Ealloc;
+  if (f32) V.v32 = (float *)dst;
+  else V.v64 = (double *)dst;
+  s.bytes += es * (size_t)V.Ealloc;
+  if (C.K) {
+    uint8_t *code = nullptr;
+    void *dict = nullptr;
+    if (dmalloc(&code, C.Ealloc) || dmalloc_void(&dict, es * C.dict_total, st)) {
+      dfree(code);
+      return fail(COSMO_ENOMEM, "tiled layout: code allocation failed");
+    }
+    HIPCHK(hipMemsetAsync(code, 0, C.Ealloc, st));  // padding slots: code 0, the value 0.0
+    HIPCHK(hipMemsetAsync(dict, 0, es * C.dict_total, st));
+    if (f32)
+      hipLaunchKernelGGL((k_sell_dict<float, unsigned int>), dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1,
+                         (const int64_t *)s.boff, (const int64_t *)M.rowptr, (const uint32_t *)M.cols,
+                         (const float *)M.v32, rid, (const uint16_t *)s.lrow, (const int32_t *)C.nbr_start,
+                         (const int32_t *)C.nbr_tile, (int32_t *)nullptr, (const int64_t *)C.qstart,
+                         (const int64_t *)C.sl_base, (const int32_t *)C.dict_off, (float *)dict, code);
+    else
+      hipLaunchKernelGGL((k_sell_dict<double, unsigned long long>), dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1,
+                         (const int64_t *)s.boff, (const int64_t *)M.rowptr, (const uint32_t *)M.cols,
+                         (const double *)M.v64, rid, (const uint16_t *)s.lrow, (const int32_t *)C.nbr_start,
+                         (const int32_t *)C.nbr_tile, (int32_t *)nullptr, (const int64_t *)C.qstart,
+                         (const int64_t *)C.sl_base, (const int32_t *)C.dict_off, (double *)dict, code);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    if (f32) C.code32 = code, C.dict32 = (float *)dict;
+    else C.code64 = code, C.dict64 = (double *)dict;
+    s.bytes += (size_t)C.Ealloc + es * (size_t)C.dict_total;
+  }
   if (!s.split) return COSMO_OK;
   void *pd = nullptr;
-  const size_t es = f32 ? sizeof(float) : sizeof(double);
   if (dmalloc_void(&pd, es * s.pm_Ealloc, st)) return fail(COSMO_ENOMEM, "tiled layout: value allocation failed");
   HIPCHK(hipMemsetAsync(pd, 0, es * s.pm_Ealloc, st));
   hipLaunchKernelGGL(k_pm_slots, dim3(s.pm_units), dim3(s.pm_nwaves * 64), 0, st, 1, (const int4 *)s.pm_items,
@@ -926,10 +1283,11 @@ static int sell_values(Op &op, bool f32, hipStream_t st) {
 
 // device bytes the tiled layout of an operator with n rows and nnz stored entries can
 // take with both value copies (the padding of the panels stays below 1.7 x on the
-// systems measured): the server's admission counts it beside the rows
+// systems measured; codes and dictionaries where pairs are coded): the server's admission
+// counts it beside the rows
 int64_t sell_footprint_bound(int64_t n, int64_t nnz) {
   if (n < SELL_MIN_N) return 0;
-  return (int64_t)(1.7 * (double)nnz) * (2 + 4 + 8) + 64 * n;
+  return (int64_t)(1.7 * (double)nnz) * (2 + 4 + 8 + 2) + 64 * n + 256 * n;
 }
 
 int g_sell_chunk = 0;  // XCD chunk of row blocks (COSMO_SELL_CHUNK; 0: chosen by timing)
@@ -937,31 +1295,69 @@ int g_sell_off = 0;    // COSMO_SELL=0: the row kernel instead
 int g_sell_stats = 0;  // COSMO_SELL_STATS=1: the layout's sizes on stderr when it is built
 int g_sell_split = 1;  // COSMO_SELL_SPLIT=0: the proton-spectator entries stay in the panels
 
-// one product: the proton-spectator part (when split) writes every row of Y, then the
-// panels add the rest of each row to it
+template <typename T>
+static const T *pick(const float *a, const double *b) {
+  if constexpr (sizeof(T) == 4) return (const T *)a;
+  else return (const T *)b;
+}
+
+static SellView panel_view(const Sell &s, const Panels &P, const uint8_t *codes) {
+  return SellView{P.maxtiles, s.nwaves * 64, P.items, P.pair_c0, P.pair_rows, P.sl_base, P.cols, P.dict_off,
+                  P.dict_n4, codes, s.lrow};
+}
+
 template <typename T, int LD>
-static int launch_sell(const Sell &s, const T *vals, const T *pm_vals, const T *X, T *Y, int m, int chunk,
-                       hipStream_t st) {
-  SellView v{s.maxtiles, s.nwaves * 64, s.items, s.pair_c0, s.pair_rows, s.sl_base, s.cols};
-  const size_t lds = 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(s);
+static size_t panels_lds(const Sell &s, const Panels &P) {
+  return 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(P.maxtiles, s.nwaves) +
+         (P.coded ? 2 * SELL_DICT * sizeof(T) : 0);
+}
+
+// one product: the proton-spectator part (when split) writes every row of Y; then one
+// panel launch adds the rest: the coded kernel (value pairs, then coded pairs) when any
+// pair is coded, else the value panels alone
+template <typename T, int LD>
+static int launch_sell(const Sell &s, const T *X, T *Y, int m, int chunk, hipStream_t st) {
+  const Panels &V = s.P[0], &C = s.P[1];
+  const bool coded = C.K > 0;
+  const size_t lds = coded ? panels_lds<T, LD>(s, C) : panels_lds<T, LD>(s, V);
   if (lds > 160 * 1024) return COSMO_EUNSUP;  // tiles and table do not fit one CU's LDS
-  auto kern = k_sell_apply<T, LD>;
-  auto pkern = k_pm_apply<T, LD>;
-  static bool attr_done = false;
-  if (!attr_done) {
-    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void *)pkern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_done = true;
-  }
+  int accum = 0;
   if (s.split) {
+    auto pkern = k_pm_apply<T, LD>;
+    static bool attr_done = false;
+    if (!attr_done) {
+      (void)hipFuncSetAttribute((const void *)pkern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_done = true;
+    }
     PmView pv{s.pm_items, s.pm_rows, s.pm_base, s.pm_cols};
     const int vpr = (int)(LD * sizeof(T) / 16), nthr = s.pm_nwaves * 64;
     const int pieces = (s.pm_maxC * vpr + nthr - 1) / nthr * nthr;
-    hipLaunchKernelGGL(pkern, dim3(s.pm_units), dim3(nthr), pm_lds_bytes(s, LD * sizeof(T)), st, pv, pm_vals, X, Y,
-                       m, pieces);
+    hipLaunchKernelGGL(pkern, dim3(s.pm_units), dim3(nthr), pm_lds_bytes(s, LD * sizeof(T)), st, pv,
+                       pick<T>(s.pm_v32, s.pm_v64), X, Y, m, pieces);
     HIPCHK(hipGetLastError());
+    accum = 1;
   }
-  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, v, vals, X, Y, m, chunk, s.split ? 1 : 0);
+  const SellView vv = panel_view(s, V, nullptr);
+  if (coded) {
+    auto kern = k_sell_apply<T, LD, true>;
+    static bool attr_done = false;
+    if (!attr_done) {
+      (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_done = true;
+    }
+    const SellView cv = panel_view(s, C, sizeof(T) == 4 ? C.code32 : C.code64);
+    hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, cv, pick<T>(C.dict32, C.dict64), vv,
+                       pick<T>(V.v32, V.v64), X, Y, m, chunk, accum);
+  } else {
+    auto kern = k_sell_apply<T, LD, false>;
+    static bool attr_done = false;
+    if (!attr_done) {
+      (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_done = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, vv, pick<T>(V.v32, V.v64), vv,
+                       pick<T>(V.v32, V.v64), X, Y, m, chunk, accum);
+  }
   HIPCHK(hipGetLastError());
   return COSMO_OK;
 }
@@ -971,8 +1367,8 @@ static int launch_sell(const Sell &s, const T *vals, const T *pm_vals, const T *
 // XCD runs which row block, never a row's summation, so every launch writes the same Y.
 static constexpr int SELL_CHUNKS[] = {1, 2, 4, 8, 16, 32};
 template <typename T, int LD>
-static int launch_sell_tuned(Sell &s, const T *vals, const T *pm_vals, const T *X, T *Y, int m, hipStream_t st) {
-  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, g_sell_chunk, st);
+static int launch_sell_tuned(Sell &s, const T *X, T *Y, int m, hipStream_t st) {
+  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, X, Y, m, g_sell_chunk, st);
   const int key = (int)sizeof(T) * 4096 + LD;
   int chunk;
   {
@@ -989,7 +1385,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *pm_vals, const T *
       for (int c : SELL_CHUNKS)
         for (int rep = 0; rep < 2; ++rep) {
           HIPCHK(hipEventRecord(e0, st));
-          const int rc = launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, c, st);
+          const int rc = launch_sell<T, LD>(s, X, Y, m, c, st);
           if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -1010,7 +1406,7 @@ static int launch_sell_tuned(Sell &s, const T *vals, const T *pm_vals, const T *
       return COSMO_OK;  // the timed launches wrote Y
     }
   }
-  return launch_sell<T, LD>(s, vals, pm_vals, X, Y, m, chunk, st);
+  return launch_sell<T, LD>(s, X, Y, m, chunk, st);
 }
 
 // Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
@@ -1036,16 +1432,16 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
   if (f32) {
     const float *x = (const float *)X;
     float *y = (float *)Y;
-    if (ld == 4) return launch_sell_tuned<float, 4>(s, s.v32, s.pm_v32, x, y, m, st);
-    if (ld == 8) return launch_sell_tuned<float, 8>(s, s.v32, s.pm_v32, x, y, m, st);
-    return launch_sell_tuned<float, 16>(s, s.v32, s.pm_v32, x, y, m, st);
+    if (ld == 4) return launch_sell_tuned<float, 4>(s, x, y, m, st);
+    if (ld == 8) return launch_sell_tuned<float, 8>(s, x, y, m, st);
+    return launch_sell_tuned<float, 16>(s, x, y, m, st);
   }
   const double *x = (const double *)X;
   double *y = (double *)Y;
-  if (ld == 2) return launch_sell_tuned<double, 2>(s, s.v64, s.pm_v64, x, y, m, st);
-  if (ld == 4) return launch_sell_tuned<double, 4>(s, s.v64, s.pm_v64, x, y, m, st);
-  if (ld == 8) return launch_sell_tuned<double, 8>(s, s.v64, s.pm_v64, x, y, m, st);
-  return launch_sell_tuned<double, 16>(s, s.v64, s.pm_v64, x, y, m, st);
+  if (ld == 2) return launch_sell_tuned<double, 2>(s, x, y, m, st);
+  if (ld == 4) return launch_sell_tuned<double, 4>(s, x, y, m, st);
+  if (ld == 8) return launch_sell_tuned<double, 8>(s, x, y, m, st);
+  return launch_sell_tuned<double, 16>(s, x, y, m, st);
 }
 
 }  // namespace cosmo
