@@ -962,6 +962,7 @@ int op_apply_dev(Op &op, const void *X, void *Y, int m, int ldx, int ldy, int pr
     if (const char *e = getenv("COSMO_SELL_CHUNK")) g_sell_chunk = std::max(1, atoi(e));
     if (const char *e = getenv("COSMO_SELL_STATS")) g_sell_stats = atoi(e);
     if (const char *e = getenv("COSMO_SELL_SPLIT")) g_sell_split = atoi(e) != 0;
+    if (const char *e = getenv("COSMO_SELL_NB")) g_sell_nb = atoi(e);
     env_done = true;
   }
   if (op.fact && prec != COSMO_PREC_FP64_ORDERED && ldx == ldy) {

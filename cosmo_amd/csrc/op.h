@@ -59,7 +59,7 @@ int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStrea
 void free_sell(Op &op);
 size_t sell_bytes(const Op &op);
 int64_t sell_footprint_bound(int64_t n, int64_t nnz);
-extern int g_sell_chunk, g_sell_off, g_sell_stats, g_sell_split;
+extern int g_sell_chunk, g_sell_off, g_sell_stats, g_sell_split, g_sell_nb;
 int op_diag_dev(Op &op, double *d_diag, hipStream_t st);
 int cast_pack(int64_t n, int m, const double *src, int lds, float *dst, int ldd, hipStream_t st);
 int cast_unpack(int64_t n, int m, const float *src, int lds, double *dst, int ldd, hipStream_t st);

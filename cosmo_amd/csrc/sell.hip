@@ -18,8 +18,8 @@
 //    and FP64 copies); a slice's panel is as long as its longest row in t, shorter rows are
 //    padded with zero values pointing at a zero row kept after each staged tile;
 //  * coded panels: the same layout for the (row block, tile) pairs whose entries take at
-//    most SELL_DICT - 1 distinct non-zero values, with a byte per slot that indexes the
-//    pair's dictionary (entry 0 is 0.0) instead of the value.  Off a row block's own tile
+//    most SELL_DICT - 1 distinct non-zero values, with one 32-bit word per slot: the column
+//    in the low half, above it the index into the pair's dictionary (entry 0 is 0.0).  Off a row block's own tile
 //    the proton-neutron entries of a pair all move the same neutron, so their values are
 //    the interaction's for that move, with signs: a hundred or so values.
 // Lanes take the rows of a block in the order of their numbers of entries (lrow), so the
@@ -47,7 +47,8 @@
 namespace cosmo {
 
 #define SELL_MAXROWS 576  // rows of a row block: nine wavefronts, one lane per row
-#define SELL_PF 4         // entry slots per lane loaded one tile ahead
+#define SELL_PF 4         // entry slots per lane loaded one tile ahead (value panels)
+#define SELL_PFC 4        // the same for coded panels, whose slots take one register each
 #define SELL_MIN_N 65536  // smaller operators keep the row kernel
 #define SELL_DICT 256     // entries of a pair's value dictionary (entry 0 is 0.0)
 #define SELL_HASH 1024    // LDS hash slots that collect a pair's distinct values
@@ -65,9 +66,9 @@ struct Panels {
   uint16_t *cols = nullptr;      // [Ealloc] column inside the tile
   float *v32 = nullptr;          // value panels: [Ealloc]
   double *v64 = nullptr;
-  bool coded = false;            // coded panels: a byte per slot into the pair's dictionary
-  uint8_t *code32 = nullptr;     // [Ealloc] codes of the FP32 values
-  uint8_t *code64 = nullptr;     // [Ealloc] codes of the FP64 values
+  bool coded = false;            // coded panels: per slot column | dictionary index << 16
+  uint32_t *pk32 = nullptr;      // [Ealloc] slots with the FP32 dictionaries' indices
+  uint32_t *pk64 = nullptr;      // [Ealloc] slots with the FP64 dictionaries' indices
   float *dict32 = nullptr;       // [dict_total]
   double *dict64 = nullptr;
   int32_t *dict_off = nullptr;   // [K] first entry of each pair's dictionary (a multiple of 4)
@@ -84,7 +85,7 @@ struct Sell {
   uint16_t *lrow = nullptr;      // [n] row of block b taken by lane l: boff[b] + lrow[boff[b] + l]
   Panels P[2];                   // value panels, coded panels
   size_t bytes = 0;
-  std::map<int, int> chunk;      // (element size, pitch) -> XCD chunk chosen by timing
+  std::map<int, int> chunk;      // (element size, pitch) -> 4 * XCD chunk + tile buffers, chosen by timing
   // Proton-spectator part: the entries between two different neutron configurations with
   // the same proton configuration (neutron two-move terms, and one-move terms whose proton
   // configuration is unchanged), multiplied on proton-major tiles by k_pm_apply and left
@@ -123,7 +124,7 @@ struct SellView {
   const uint16_t *cols;
   const int32_t *dict_off;       // coded panels: per pair its dictionary
   const int32_t *dict_n4;
-  const uint8_t *codes;
+  const uint32_t *packed;        // coded panels: column | dictionary index << 16
   const uint16_t *lrow;
 };
 
@@ -330,15 +331,15 @@ __device__ __forceinline__ uint32_t key_hash(unsigned long long k) {
 // an LDS hash table (one workgroup per row block, one lane per row, pairs in turn).
 // pass 0: count[k] = their number, or SELL_DICT when there are more than SELL_DICT - 1;
 // pass 1 (pairs of at most SELL_DICT - 1 values): the pair's dictionary at dict_off[k]
-// (entry 0 is 0.0, then the values in hash-table order) and every slot's code, padding
-// slots keeping code 0.
+// (entry 0 is 0.0, then the values in hash-table order) and every slot's word (column
+// inside the tile | index << 16); padding slots keep the zero row and index 0.
 template <typename T, typename KT>
 __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_dict(
     int pass, const int64_t *__restrict__ boff, const int64_t *__restrict__ rowptr,
     const uint32_t *__restrict__ cols, const T *__restrict__ vals, const uint64_t *__restrict__ rid,
     const uint16_t *__restrict__ lrow, const int32_t *__restrict__ nbr_start, const int32_t *__restrict__ nbr_tile,
     int32_t *__restrict__ count, const int64_t *__restrict__ qstart, const int64_t *__restrict__ sl_base,
-    const int32_t *__restrict__ dict_off, T *__restrict__ dict, uint8_t *__restrict__ code) {
+    const int32_t *__restrict__ dict_off, T *__restrict__ dict, uint32_t *__restrict__ packed) {
   __shared__ KT tab[SELL_HASH];
   __shared__ uint8_t tcode[SELL_HASH];
   __shared__ int nkeys, over;
@@ -407,13 +408,13 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_dict(
       for (int64_t x = e_first; x < e; ++x) {
         if (pm_entry(rid, ra, cols[x])) continue;
         const KT key = val_bits(vals[x]);
-        uint8_t cd = 0;
+        uint32_t cd = 0;
         if (key != 0) {  // present: every key of the pair was inserted above
           uint32_t h = key_hash(key) & (SELL_HASH - 1);
           while (tab[h] != key) h = (h + 1) & (SELL_HASH - 1);
           cd = tcode[h];
         }
-        code[base + (int64_t)c * 64] = cd;
+        packed[base + (int64_t)c * 64] = (cols[x] - cs) | (cd << 16);
         ++c;
       }
     }
@@ -454,12 +455,13 @@ __host__ __device__ inline size_t sell_meta_bytes(int maxtiles, int nwaves) {
 // A: the panels multiplied through the pipeline, with vals their slot values (CODED
 // false) or their dictionaries (CODED true); B and bvals (CODED only): the value panels,
 // multiplied first, pair by pair
-template <typename T, int LD, bool CODED>
-__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals, SellView B,
-                                                            const T *__restrict__ bvals, const T *__restrict__ X,
-                                                            T *__restrict__ Y, int m, int chunk, int accum) {
+template <typename T, int LD, bool CODED, int NB>
+__device__ __forceinline__ void sell_apply_body(const SellView &A, const T *__restrict__ vals, const SellView &B,
+                                                const T *__restrict__ bvals, const T *__restrict__ X,
+                                                T *__restrict__ Y, int m, int chunk, int accum) {
   typedef typename V16t<T>::type V;
   typedef typename std::conditional<CODED, uint32_t, T>::type W;  // what a slot holds
+  constexpr int PF = CODED ? SELL_PFC : SELL_PF;                   // slots per prefetch group
   constexpr int VPR = (int)(LD * sizeof(T) / 16);  // 16-byte pieces per X row
   constexpr int VW = (int)(16 / sizeof(T));        // elements per piece
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -476,15 +478,15 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   // 16-byte pieces, dictionary offset and length) and its slice offsets (64-slot units from
   // the block's first slot), then (coded) two dictionary buffers
   const int zrow = A.nthr;
-  V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(zrow + 1) * VPR;
-  uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf1 + (size_t)(zrow + 1) * VPR);
+  V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(NB - 1) * (zrow + 1) * VPR;
+  uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf0 + (size_t)NB * (zrow + 1) * VPR);
   int32_t *mt_np = reinterpret_cast<int32_t *>(mt_c0 + A.maxtiles);
   int32_t *mt_doff = mt_np + A.maxtiles;
   int32_t *mt_dn = mt_doff + A.maxtiles;
   uint16_t *mt_off = reinterpret_cast<uint16_t *>(mt_dn + A.maxtiles);
   T *dict0 = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(mt_c0) +
                                    sell_meta_bytes(A.maxtiles, A.nthr >> 6));
-  T *dict1 = dict0 + SELL_DICT;
+  T *dict1 = dict0 + (NB - 1) * SELL_DICT;
   const int64_t slot0 = A.sl_base[qb];
   for (int i = tid; i < K; i += nthr) {
     mt_c0[i] = A.pair_c0[k0 + i];
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     }
   }
   for (int i = tid; i <= K * ns; i += nthr) mt_off[i] = (uint16_t)((A.sl_base[qb + i] - slot0) >> 6);
-  if (tid < 2 * VPR) buf0[(size_t)(tid / VPR) * (zrow + 1) * VPR + (size_t)zrow * VPR + tid % VPR] = V{};
+  if (tid < NB * VPR) buf0[(size_t)(tid / VPR) * (zrow + 1) * VPR + (size_t)zrow * VPR + tid % VPR] = V{};
   // instruction p of an entry reads piece (p + lane) % VPR of its X row into accumulator
   // block p: lanes on random rows then hit all LDS banks (a plain piece-p read from rows
   // 64 bytes apart reaches a quarter of them); the rotation is undone when Y is written
@@ -545,22 +547,26 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     if (CODED) dma_dict(mt_doff[kk], mt_dn[kk], dd);
   };
   // the panel of this wavefront's slice for step kk (the same for the whole wavefront:
-  // held in scalar registers); slots [j0, j0 + SELL_PF) of a lane are read unconditionally
-  // (the slot arrays carry SELL_PF * 64 slots of slack), so the compiler never makes a
+  // held in scalar registers); slots [j0, j0 + PF) of a lane are read unconditionally
+  // (the slot arrays carry SELL_PFC * 64 slots of slack), so the compiler never makes a
   // load wait for the next tile's loads at a branch
   auto slice_of = [&](int kk, int64_t &base, int &L) {
     const int o = __builtin_amdgcn_readfirstlane((int)mt_off[kk * ns + wc]);
     base = slot0 + (int64_t)o * 64 + lane;
     L = __builtin_amdgcn_readfirstlane((int)mt_off[kk * ns + wc + 1]) - o;
   };
-  auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[SELL_PF], W (&h)[SELL_PF]) {
+  auto load_slots = [&](int64_t base, int L, int j0, uint32_t (&c)[PF], W (&h)[PF]) {
     (void)L;
 #pragma unroll
-    for (int j = 0; j < SELL_PF; ++j) {
+    for (int j = 0; j < PF; ++j) {
       const int64_t p = base + (int64_t)(j0 + j) * 64;
-      c[j] = A.cols[p];
-      if constexpr (CODED) h[j] = A.codes[p];
-      else h[j] = vals[p];
+      if constexpr (CODED) {
+        c[j] = A.packed[p];
+        (void)h;
+      } else {
+        c[j] = A.cols[p];
+        h[j] = vals[p];
+      }
     }
   };
   auto fma_x = [&](const V *tile, uint32_t c, T h) {
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     }
   };
   auto fma_row = [&](const V *tile, const T *dict, uint32_t c, W hw) {
-    if constexpr (CODED) fma_x(tile, c, dict[hw]);
+    if constexpr (CODED) fma_x(tile, c & 0xffffu, dict[c >> 16]), (void)hw;
     else fma_x(tile, c, hw), (void)dict;
   };
   // One tile step: the next tile's X pieces and the first slot group of its panel are
@@ -582,19 +588,19 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   // step loop is unrolled by two so the sets trade roles instead of being copied: a copy
   // of a register whose load is in flight makes the compiler wait for every load issued
   // before it, the next tile's included.
-  uint32_t c0s[SELL_PF], c1s[SELL_PF], cx[SELL_PF];
-  W h0s[SELL_PF], h1s[SELL_PF], hx[SELL_PF];
+  uint32_t c0s[PF], c1s[PF], cx[PF];
+  W h0s[PF], h1s[PF], hx[PF];
   int64_t base0 = 0, base1 = 0;
   int L0 = 0, L1 = 0;
   // slots j0 + j past the panel's end L are skipped here at the use (a branch uniform over
   // the wavefront), not at the load, so the loads stay in flight until the group is reached
-  auto group = [&](const V *cur, const T *cd, const uint32_t (&c)[SELL_PF], const W (&h)[SELL_PF], int j0, int L) {
+  auto group = [&](const V *cur, const T *cd, const uint32_t (&c)[PF], const W (&h)[PF], int j0, int L) {
 #pragma unroll
-    for (int j = 0; j < SELL_PF; ++j)
+    for (int j = 0; j < PF; ++j)
       if (j0 + j < L) fma_row(cur, cd, c[j], h[j]);
   };
-  auto step = [&](int kk, const V *cur, const T *cd, V *nxt, T *nd, uint32_t (&ca)[SELL_PF], W (&ha)[SELL_PF],
-                  int64_t base_a, int La, uint32_t (&cb)[SELL_PF], W (&hb)[SELL_PF], int64_t &base_b, int &Lb) {
+  auto step = [&](int kk, const V *cur, const T *cd, V *nxt, T *nd, uint32_t (&ca)[PF], W (&ha)[PF],
+                  int64_t base_a, int La, uint32_t (&cb)[PF], W (&hb)[PF], int64_t &base_b, int &Lb) {
     // the next step's loads are issued unconditionally (the last step reloads its own tile
     // and slots, unused): a load under a branch would make the first use after the join
     // wait for every load in flight, the next tile's included
@@ -602,32 +608,38 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
     const int kn = more ? kk + 1 : kk;
     slice_of(kn, base_b, Lb);
     load_slots(base_b, Lb, 0, cb, hb);
-    if (more) dma_tile(kn, nxt, nd);  // the buffers read in the previous step: free since its barrier
+    if (NB == 2 && more) dma_tile(kn, nxt, nd);  // the buffers read in the previous step: free since its barrier
     if (wave_on && La > 0) {
       group(cur, cd, ca, ha, 0, La);
-      if (SELL_PF < La) {
-        load_slots(base_a, La, SELL_PF, cx, hx);
-        for (int j0 = SELL_PF; j0 < La; j0 += 2 * SELL_PF) {
-          if (j0 + SELL_PF < La) load_slots(base_a, La, j0 + SELL_PF, ca, ha);
+      if (PF < La) {
+        load_slots(base_a, La, PF, cx, hx);
+        for (int j0 = PF; j0 < La; j0 += 2 * PF) {
+          if (j0 + PF < La) load_slots(base_a, La, j0 + PF, ca, ha);
           group(cur, cd, cx, hx, j0, La);
-          if (j0 + SELL_PF >= La) break;
-          if (j0 + 2 * SELL_PF < La) load_slots(base_a, La, j0 + 2 * SELL_PF, cx, hx);
-          group(cur, cd, ca, ha, j0 + SELL_PF, La);
+          if (j0 + PF >= La) break;
+          if (j0 + 2 * PF < La) load_slots(base_a, La, j0 + 2 * PF, cx, hx);
+          group(cur, cd, ca, ha, j0 + PF, La);
         }
       }
+    }
+    if (NB == 1) {  // one buffer: the next tile is loaded once every wavefront is done with it
+      __syncthreads();
+      if (more) dma_tile(kn, nxt, nd);
     }
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wavefront's direct loads have landed
     __syncthreads();
   };
   // the first pipelined tile (and its dictionary) straight from memory, beside the table's
-  // loads; it lands while the value pairs (coded kernel) are multiplied in the other buffer
-  if (K > 0) {
+  // loads; with two buffers it lands while the value pairs (coded kernel) are multiplied in
+  // the other one
+  auto first_tile = [&]() {
     dma(A.pair_c0[k0], A.pair_rows[k0] * VPR, buf0);
     if (CODED) dma_dict(A.dict_off[k0], A.dict_n4[k0], dict0);
-  }
+  };
+  if (NB == 2 && K > 0) first_tile();
   if constexpr (CODED) {
-    // value pairs one at a time in buf1: tile, wait, multiply; slot groups of SELL_PF with
-    // the next group in flight
+    // value pairs one at a time in buf1 (buf0 with one buffer): tile, wait, multiply; slot
+    // groups of SELL_PF with the next group in flight
     const int4 e0 = B.items[2 * b], e1 = B.items[2 * b + 1];
     const int bk0 = e0.x, bK = e0.y;
     uint32_t cva[SELL_PF], cvb[SELL_PF];
@@ -667,6 +679,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
   }
   if (K > 0) {
     // the first slots; the barrier after them also publishes the table
+    if (NB == 1) first_tile();
     const int64_t q0 = A.sl_base[qb + wc];
     base0 = q0 + lane;
     L0 = __builtin_amdgcn_readfirstlane((int)((A.sl_base[qb + wc + 1] - q0) >> 6));
@@ -695,6 +708,13 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T
       }
     }
   }
+}
+
+template <typename T, int LD, bool CODED, int NB>
+__global__ __launch_bounds__(SELL_MAXROWS) void k_sell_apply(SellView A, const T *__restrict__ vals, SellView B,
+                                                            const T *__restrict__ bvals, const T *__restrict__ X,
+                                                            T *__restrict__ Y, int m, int chunk, int accum) {
+  sell_apply_body<T, LD, CODED, NB>(A, vals, B, bvals, X, Y, m, chunk, accum);
 }
 
 // Proton-spectator part of Y = H X: one workgroup per unit (a proton configuration and
@@ -809,7 +829,7 @@ void free_sell(Op &op) {
   if (!s) return;
   for (Panels &P : s->P)
     for (void *p : {(void *)P.nbr_start, (void *)P.nbr_tile, (void *)P.qstart, (void *)P.sl_base, (void *)P.cols,
-                    (void *)P.v32, (void *)P.v64, (void *)P.code32, (void *)P.code64, (void *)P.dict32,
+                    (void *)P.v32, (void *)P.v64, (void *)P.pk32, (void *)P.pk64, (void *)P.dict32,
                     (void *)P.dict64, (void *)P.dict_off, (void *)P.dict_n4, (void *)P.pair_c0,
                     (void *)P.pair_rows, (void *)P.items})
       dfree(p);
@@ -1001,7 +1021,7 @@ static int build_panels(Sell &s, Panels &P, const std::vector<int64_t> &hb, cons
     const int64_t qe = hq[b] + (int64_t)(hs[b + 1] - hs[b]) * ((hb[b + 1] - hb[b] + 63) / 64);
     if ((hbase[qe] - hbase[hq[b]]) / 64 > 65535) return fail(COSMO_EUNSUP, "a row block too long for the tiled product");
   }
-  P.Ealloc = P.E + 64 * SELL_PF;  // a lane reads up to SELL_PF slots past its slice's panel
+  P.Ealloc = P.E + 64 * SELL_PFC;  // a lane reads up to SELL_PFC slots past its slice's panel
   if (dmalloc(&P.cols, sizeof(uint16_t) * P.Ealloc)) return fail(COSMO_ENOMEM, "tiled layout: allocation failed");
   HIPCHK(hipMemcpyAsync(P.sl_base, hbase.data(), sizeof(int64_t) * (Q + 1), hipMemcpyHostToDevice, st));
   // padding slots point at the zero row kept after every staged tile (index nwaves * 64)
@@ -1156,7 +1176,7 @@ static int build_sell(Op &op, hipStream_t st) {
                      (const int64_t *)s->boff, (const int64_t *)M.rowptr, (const uint32_t *)M.cols,
                      (const double *)M.v64, rid, (const uint16_t *)s->lrow, (const int32_t *)all_start,
                      (const int32_t *)all_tile, dcnt, (const int64_t *)nullptr, (const int64_t *)nullptr,
-                     (const int32_t *)nullptr, (double *)nullptr, (uint8_t *)nullptr);
+                     (const int32_t *)nullptr, (double *)nullptr, (uint32_t *)nullptr);
   HIPCHK(hipGetLastError());
   std::vector<int32_t> ht(std::max<int64_t>(K, 1)), hd(std::max<int64_t>(K, 1), SELL_DICT);
   if (K) {
@@ -1238,13 +1258,14 @@ static int sell_values(Op &op, bool f32, hipStream_t st) {
   else V.v64 = (double *)dst;
   s.bytes += es * (size_t)V.Ealloc;
   if (C.K) {
-    uint8_t *code = nullptr;
+    uint32_t *code = nullptr;
     void *dict = nullptr;
-    if (dmalloc(&code, C.Ealloc) || dmalloc_void(&dict, es * C.dict_total, st)) {
+    if (dmalloc(&code, sizeof(uint32_t) * C.Ealloc) || dmalloc_void(&dict, es * C.dict_total, st)) {
       dfree(code);
       return fail(COSMO_ENOMEM, "tiled layout: code allocation failed");
     }
-    HIPCHK(hipMemsetAsync(code, 0, C.Ealloc, st));  // padding slots: code 0, the value 0.0
+    // padding slots: the zero row, dictionary entry 0 (the value 0.0)
+    HIPCHK(hipMemsetD32Async((hipDeviceptr_t)code, (int)(s.nwaves * 64), C.Ealloc, st));
     HIPCHK(hipMemsetAsync(dict, 0, es * C.dict_total, st));
     if (f32)
       hipLaunchKernelGGL((k_sell_dict<float, unsigned int>), dim3(s.nblk), dim3(s.nwaves * 64), 0, st, 1,
@@ -1260,9 +1281,9 @@ static int sell_values(Op &op, bool f32, hipStream_t st) {
                          (const int64_t *)C.sl_base, (const int32_t *)C.dict_off, (double *)dict, code);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    if (f32) C.code32 = code, C.dict32 = (float *)dict;
-    else C.code64 = code, C.dict64 = (double *)dict;
-    s.bytes += (size_t)C.Ealloc + es * (size_t)C.dict_total;
+    if (f32) C.pk32 = code, C.dict32 = (float *)dict;
+    else C.pk64 = code, C.dict64 = (double *)dict;
+    s.bytes += sizeof(uint32_t) * (size_t)C.Ealloc + es * (size_t)C.dict_total;
   }
   if (!s.split) return COSMO_OK;
   void *pd = nullptr;
@@ -1301,25 +1322,39 @@ static const T *pick(const float *a, const double *b) {
   else return (const T *)b;
 }
 
-static SellView panel_view(const Sell &s, const Panels &P, const uint8_t *codes) {
+static SellView panel_view(const Sell &s, const Panels &P, const uint32_t *packed) {
   return SellView{P.maxtiles, s.nwaves * 64, P.items, P.pair_c0, P.pair_rows, P.sl_base, P.cols, P.dict_off,
-                  P.dict_n4, codes, s.lrow};
+                  P.dict_n4, packed, s.lrow};
 }
 
 template <typename T, int LD>
-static size_t panels_lds(const Sell &s, const Panels &P) {
-  return 2 * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(P.maxtiles, s.nwaves) +
-         (P.coded ? 2 * SELL_DICT * sizeof(T) : 0);
+static size_t panels_lds(const Sell &s, const Panels &P, int nb) {
+  return nb * (size_t)(s.nwaves * 64 + 1) * LD * sizeof(T) + sell_meta_bytes(P.maxtiles, s.nwaves) +
+         (P.coded ? nb * SELL_DICT * sizeof(T) : 0);
+}
+
+int g_sell_nb = 0;  // COSMO_SELL_NB: tile buffers per workgroup of the panel product (1, 2; 0: by timing)
+
+template <typename T, int LD, bool CODED, int NB>
+static void launch_panel_kernel(const Sell &s, size_t lds, const SellView &a, const T *avals, const SellView &b,
+                                const T *bvals, const T *X, T *Y, int m, int chunk, int accum, hipStream_t st) {
+  auto kern = k_sell_apply<T, LD, CODED, NB>;
+  static bool attr_done = false;
+  if (!attr_done) {
+    (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_done = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, a, avals, b, bvals, X, Y, m, chunk, accum);
 }
 
 // one product: the proton-spectator part (when split) writes every row of Y; then one
 // panel launch adds the rest: the coded kernel (value pairs, then coded pairs) when any
 // pair is coded, else the value panels alone
 template <typename T, int LD>
-static int launch_sell(const Sell &s, const T *X, T *Y, int m, int chunk, hipStream_t st) {
+static int launch_sell(const Sell &s, const T *X, T *Y, int m, int chunk, int nb, hipStream_t st) {
   const Panels &V = s.P[0], &C = s.P[1];
   const bool coded = C.K > 0;
-  const size_t lds = coded ? panels_lds<T, LD>(s, C) : panels_lds<T, LD>(s, V);
+  const size_t lds = coded ? panels_lds<T, LD>(s, C, nb) : panels_lds<T, LD>(s, V, nb);
   if (lds > 160 * 1024) return COSMO_EUNSUP;  // tiles and table do not fit one CU's LDS
   int accum = 0;
   if (s.split) {
@@ -1338,75 +1373,78 @@ static int launch_sell(const Sell &s, const T *X, T *Y, int m, int chunk, hipStr
     accum = 1;
   }
   const SellView vv = panel_view(s, V, nullptr);
+  const T *vvals = pick<T>(V.v32, V.v64);
   if (coded) {
-    auto kern = k_sell_apply<T, LD, true>;
-    static bool attr_done = false;
-    if (!attr_done) {
-      (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr_done = true;
-    }
-    const SellView cv = panel_view(s, C, sizeof(T) == 4 ? C.code32 : C.code64);
-    hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, cv, pick<T>(C.dict32, C.dict64), vv,
-                       pick<T>(V.v32, V.v64), X, Y, m, chunk, accum);
+    const SellView cv = panel_view(s, C, sizeof(T) == 4 ? C.pk32 : C.pk64);
+    const T *dict = pick<T>(C.dict32, C.dict64);
+    if (nb == 1) launch_panel_kernel<T, LD, true, 1>(s, lds, cv, dict, vv, vvals, X, Y, m, chunk, accum, st);
+    else launch_panel_kernel<T, LD, true, 2>(s, lds, cv, dict, vv, vvals, X, Y, m, chunk, accum, st);
   } else {
-    auto kern = k_sell_apply<T, LD, false>;
-    static bool attr_done = false;
-    if (!attr_done) {
-      (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr_done = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(s.nblk), dim3(s.nwaves * 64), lds, st, vv, pick<T>(V.v32, V.v64), vv,
-                       pick<T>(V.v32, V.v64), X, Y, m, chunk, accum);
+    if (nb == 1) launch_panel_kernel<T, LD, false, 1>(s, lds, vv, vvals, vv, vvals, X, Y, m, chunk, accum, st);
+    else launch_panel_kernel<T, LD, false, 2>(s, lds, vv, vvals, vv, vvals, X, Y, m, chunk, accum, st);
   }
   HIPCHK(hipGetLastError());
   return COSMO_OK;
 }
 
-// One product with the XCD chunk of this (precision, pitch): on the first call every chunk
-// of SELL_CHUNKS is timed (twice each) and the fastest kept.  The chunk only changes which
-// XCD runs which row block, never a row's summation, so every launch writes the same Y.
+// One product with the XCD chunk and tile buffers of this (precision, pitch): on the first
+// call every chunk of SELL_CHUNKS with one and with two tile buffers is timed (twice each)
+// and the fastest kept.  One buffer leaves room in LDS for more workgroups per CU, two
+// overlap each tile's load with the previous tile's products.  Neither changes a row's
+// summation, so every launch writes the same Y.
 static constexpr int SELL_CHUNKS[] = {1, 2, 4, 8, 16, 32};
 template <typename T, int LD>
 static int launch_sell_tuned(Sell &s, const T *X, T *Y, int m, hipStream_t st) {
-  if (g_sell_chunk > 0) return launch_sell<T, LD>(s, X, Y, m, g_sell_chunk, st);
+  const int nb_only = (g_sell_nb == 1 || g_sell_nb == 2) ? g_sell_nb : 0;
+  if (g_sell_chunk > 0 && nb_only) return launch_sell<T, LD>(s, X, Y, m, g_sell_chunk, nb_only, st);
   const int key = (int)sizeof(T) * 4096 + LD;
-  int chunk;
+  int chunk, nb;
   {
     std::lock_guard<std::mutex> lk(g_sell_mu);
     auto it = s.chunk.find(key);
     if (it != s.chunk.end()) {
-      chunk = it->second;
+      chunk = it->second / 4;
+      nb = it->second % 4;
     } else {
       hipEvent_t e0, e1;
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
       float best_ms = 1e30f;
-      chunk = 8;
-      for (int c : SELL_CHUNKS)
-        for (int rep = 0; rep < 2; ++rep) {
-          HIPCHK(hipEventRecord(e0, st));
-          const int rc = launch_sell<T, LD>(s, X, Y, m, c, st);
-          if (rc) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return rc;
+      chunk = 0;
+      nb = 2;
+      for (int b : {1, 2}) {
+        if (nb_only && b != nb_only) continue;
+        for (int c : SELL_CHUNKS) {
+          if (g_sell_chunk > 0 && c != g_sell_chunk) continue;
+          for (int rep = 0; rep < 2; ++rep) {
+            HIPCHK(hipEventRecord(e0, st));
+            const int rc = launch_sell<T, LD>(s, X, Y, m, c, b, st);
+            if (rc == COSMO_EUNSUP) break;  // this buffer count does not fit LDS
+            if (rc) {
+              (void)hipEventDestroy(e0);
+              (void)hipEventDestroy(e1);
+              return rc;
+            }
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (t < best_ms) best_ms = t, chunk = c, nb = b;
           }
-          HIPCHK(hipEventRecord(e1, st));
-          HIPCHK(hipEventSynchronize(e1));
-          float t = 0;
-          HIPCHK(hipEventElapsedTime(&t, e0, e1));
-          if (t < best_ms) best_ms = t, chunk = c;
         }
+      }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
-      s.chunk.emplace(key, chunk);
+      if (chunk == 0) return COSMO_EUNSUP;  // nothing fits LDS: the caller falls back
+      s.chunk.emplace(key, chunk * 4 + nb);
       if (g_sell_stats)
-        fprintf(stderr, "[sell] XCD chunk %d for %d-byte values at pitch %d (%.3f ms)\n", chunk, (int)sizeof(T), LD,
-                best_ms);
-      return COSMO_OK;  // the timed launches wrote Y
+        fprintf(stderr, "[sell] XCD chunk %d, %d tile buffer(s) for %d-byte values at pitch %d (%.3f ms)\n", chunk, nb,
+                (int)sizeof(T), LD, best_ms);
+      // the fastest configuration writes Y last, from the same X
+      return launch_sell<T, LD>(s, X, Y, m, chunk, nb, st);
     }
   }
-  return launch_sell<T, LD>(s, X, Y, m, chunk, st);
+  return launch_sell<T, LD>(s, X, Y, m, chunk, nb, st);
 }
 
 // Y = H X through the tiled layout when it applies (stored operator, row pitch ld with
