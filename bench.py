@@ -172,7 +172,9 @@ def main():
 
     # X and Y are dense n x k, so the row pitch is k: the tiled kernel serves pitches 4, 8, 16
     tiled = k in (4, 8, 16) and (args.layout == "tiled" or (args.layout == "auto" and n >= 65536))
-    kname = "k_sell_apply<float,%d>" % k if tiled else "k_spmm_vec<float,4>"
+    # the tiled stored product is two launches per H.V: the proton-spectator part
+    # (k_pm_apply) and the panels (k_sell_apply); the events and the traffic cover both
+    kname = ("k_pm_apply<float,%d> + k_sell_apply<float,%d>" % (k, k)) if tiled else "k_spmm_vec<float,4>"
     kfile = ("k_sell_apply" if tiled else "k_spmm_vec") + "_%s_k%d_traffic.json" % (name, k)
     traffic = None
     tj = args.traffic_json or os.path.join(ROOT, "profiles", kfile)
