@@ -207,6 +207,7 @@ def main():
     if not args.no_solve:
         out["solve"] = solve_section(ca, fop if fop is not None else op, op, n)
         out["solve_sd6p6n"] = solve_sd6p6n_section(ca, synth)
+        out["solve_sd4p4n"] = solve_sd4p4n_section(ca, synth)
         if fop is not None:
             out["solve_ref16"] = solve_ref16_section(ca, fop)
     del X, Y
@@ -378,10 +379,52 @@ def solve_sd6p6n_section(ca, synth, lims=(30, 90)):
         ref = None
     if ref is not None:
         e_ref = np.array(ref.pop("evals"))
+        ref.pop("resid")
         ref["max_abs_dE_vs_gpu_fp64_MeV"] = float(np.max(np.abs(e_ref - raw[("fp64", best["fp64"])])))
         # Davidson time alone on both sides (H built beforehand on both)
         ref["gpu_mixed_speedup"] = round(ref["t_davidson_s"] / r["mixed_lim%d" % best["mixed"]]["seconds"], 1)
         ref["gpu_fp64_speedup"] = round(ref["t_davidson_s"] / r["fp64_lim%d" % best["fp64"]]["seconds"], 1)
+        r["cpu_reference"] = ref
+    return r
+
+
+def solve_sd4p4n_section(ca, synth):
+    """The plain CPU reference workload: sd shell 4p4n M=0 (n = 28,503, seed 0), FP64 CSB H,
+    the lowest 10 pairs with the reference driver's settings (block = states = 10, basis
+    limit 30, critr 1e-8), on the GPU in FP64 and in mixed precision, beside the reference's
+    own build and Davidson (PPMatrix2CSBBlk + DavidsonDiagonalization, OpenMP) on this host."""
+    import numpy as np
+    tdir = tempfile.mkdtemp(prefix="cosmo_sd4_")
+    s = ca.System.from_xml(synth.write_system(tdir, "sd", "sd4p4n", 4, 4, 0, "+", 0))
+    t0 = time.perf_counter()
+    op = ca.Operator.build(s)
+    t_build = time.perf_counter() - t0
+    ca.davidson(op, 10, want_vectors=False, precision=ca.PREC_FP32)  # warm
+    r, raw = {"n": int(s.dim), "block": 10, "pairs": 10, "lim": 30, "critr": 1e-8, "gpu_build_s": round(t_build, 3)}, {}
+    for label, prec in (("fp64", ca.PREC_FP64), ("mixed", ca.PREC_FP32)):
+        t0 = time.perf_counter()
+        ev, de, rs, _, st = ca.davidson(op, 10, want_vectors=False, precision=prec)
+        dt = time.perf_counter() - t0
+        raw[label] = np.array(ev)
+        r[label] = dict(seconds=round(dt, 3), max_resid=float(np.max(rs)), iterations=st["iterations"],
+                        matvecs=st["matvecs"], matvecs_fp64=st["matvecs_fp64"], converged=st["converged"])
+        print("[bench] solve sd4p4n %s: %.3fs iters %d resid %.2e" % (label, dt, st["iterations"], np.max(rs)),
+              file=sys.stderr, flush=True)
+    r["max_abs_dE_mixed_vs_fp64_MeV"] = float(np.max(np.abs(raw["mixed"] - raw["fp64"])))
+    op.close()
+    try:  # a side measurement: it never stops the bench
+        ref = cpu_reference_solve(tdir, "sd4p4n", 10)
+    except Exception as e:  # noqa: BLE001
+        print("[bench] cpu reference solve failed: %r" % (e,), file=sys.stderr, flush=True)
+        ref = None
+    if ref is not None:
+        e_ref = np.array(ref.pop("evals"))
+        # pairs the reference left unconverged (residual above 1e-6) are not compared
+        ok = [i for i, x in enumerate(ref.pop("resid")) if x <= 1e-6]
+        ref["compared_pairs"] = len(ok)
+        ref["max_abs_dE_vs_gpu_fp64_MeV"] = float(np.max(np.abs(e_ref[ok] - raw["fp64"][ok]))) if ok else None
+        ref["gpu_fp64_speedup"] = round(ref["t_davidson_s"] / r["fp64"]["seconds"], 1)
+        ref["gpu_mixed_speedup"] = round(ref["t_davidson_s"] / r["mixed"]["seconds"], 1)
         r["cpu_reference"] = ref
     return r
 
@@ -415,7 +458,8 @@ def cpu_reference_solve(d, name, bloc):
           file=sys.stderr, flush=True)
     return {"cores": cores, "block": bloc, "pairs": bloc, "critr": 1e-8, "t_build_s": round(info["t_build"], 3),
             "t_davidson_s": round(info["t_davidson"], 3), "wall_s": round(wall, 3),
-            "max_resid": float(np.max(R)) if len(R) else None, "evals": [float(x) for x in E[:bloc]]}
+            "max_resid": float(np.max(R)) if len(R) else None, "evals": [float(x) for x in E[:bloc]],
+            "resid": [float(x) for x in R[:bloc]]}
 
 
 def batch_section(ca, synth):
