@@ -113,6 +113,38 @@ __device__ __forceinline__ bool pm_entry(const uint64_t *__restrict__ rid, uint6
   return (uint32_t)ra == (uint32_t)rb && (ra >> 32) != (rb >> 32);
 }
 
+#define SELL_ORDER_MAX 64  // entries of a lane in one panel that panel_order places
+
+// The slot of each of a lane's entries in one panel.  In the product, lanes l, l+4, l+8 and
+// l+12 read the same piece position of their X rows at once (the pieces rotate with the
+// lane), so their reads fall in different LDS bank groups when their rows inside the tile
+// differ modulo 4.  At slot j a lane therefore takes, while it has one, an entry whose row
+// inside the tile is (j + l / 4) modulo 4, else its first entry left in column order.
+// slot[k] receives the slot of the lane's k-th entry (proton-spectator entries skipped);
+// false when the lane has more than SELL_ORDER_MAX entries there (column order then).
+__device__ inline bool panel_order(const uint32_t *__restrict__ cols, const uint64_t *__restrict__ rid, uint64_t ra,
+                                   int64_t e0, int64_t e1, uint32_t cs, int lane, uint8_t *slot) {
+  uint8_t cls[SELL_ORDER_MAX];
+  int cnt = 0;
+  for (int64_t e = e0; e < e1; ++e) {
+    if (pm_entry(rid, ra, cols[e])) continue;
+    if (cnt == SELL_ORDER_MAX) return false;
+    cls[cnt++] = (uint8_t)((cols[e] - cs) & 3u);
+  }
+  uint64_t used = 0;
+  for (int j = 0; j < cnt; ++j) {
+    const int want = (j + (lane >> 2)) & 3;
+    int pick = -1;
+    for (int k = 0; k < cnt && pick < 0; ++k)
+      if (!((used >> k) & 1ull) && cls[k] == want) pick = k;
+    for (int k = 0; k < cnt && pick < 0; ++k)
+      if (!((used >> k) & 1ull)) pick = k;
+    used |= 1ull << pick;
+    slot[pick] = (uint8_t)j;
+  }
+  return true;
+}
+
 // A row block's item is two int4: (first pair, tiles, rows, 0) and (first row, first
 // slice, 0, 0); pairs are the (row block, column tile) pairs in block-major order.
 struct SellView {
@@ -216,9 +248,13 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_slots(
       if (lane == 0) len[q] = c;
     } else {
       const int64_t base = sl_base[q] + lane;
-      for (; e < ee && cols[e] < ce; ++e) {
+      int64_t e_end = e;
+      while (e_end < ee && cols[e_end] < ce) ++e_end;
+      uint8_t slot[SELL_ORDER_MAX];
+      const bool ordered = panel_order(cols, rid, ra, e, e_end, cs, lane, slot);
+      for (; e < e_end; ++e) {
         if (pm_entry(rid, ra, cols[e])) continue;
-        const int64_t p = base + (int64_t)c * 64;
+        const int64_t p = base + (int64_t)(ordered ? slot[c] : c) * 64;
         if (scol) scol[p] = (uint16_t)(cols[e] - cs);
         if (s64) s64[p] = v64[e];
         if (s32) s32[p] = v32[e];
@@ -404,6 +440,8 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_dict(
     __syncthreads();
     if (on) {
       const int64_t base = sl_base[qstart[b] + (int64_t)(k - k0) * ns + w] + lane;
+      uint8_t slot[SELL_ORDER_MAX];
+      const bool ordered = panel_order(cols, rid, ra, e_first, e, cs, lane, slot);
       int c = 0;
       for (int64_t x = e_first; x < e; ++x) {
         if (pm_entry(rid, ra, cols[x])) continue;
@@ -414,7 +452,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_sell_dict(
           while (tab[h] != key) h = (h + 1) & (SELL_HASH - 1);
           cd = tcode[h];
         }
-        packed[base + (int64_t)c * 64] = (cols[x] - cs) | (cd << 16);
+        packed[base + (int64_t)(ordered ? slot[c] : c) * 64] = (cols[x] - cs) | (cd << 16);
         ++c;
       }
     }
