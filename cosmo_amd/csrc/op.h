@@ -37,6 +37,7 @@ struct Op {
   hipStream_t home = nullptr;  // library stream of the creating thread
   bool foreign = false;        // used on another stream: destroy synchronises the device
   int layout = 0;              // stored product: 0 automatic, 1 rows, 2 tiled (sell.hip)
+  int64_t row_products = 0;    // automatic layout: products made with the row kernel so far
   double build_ms = 0, last_apply_ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };

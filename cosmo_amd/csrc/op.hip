@@ -209,6 +209,8 @@ int cosmo_op_create(cosmo_system s, int flags, cosmo_op *out) {
   if (!s || !out) return fail(COSMO_EINVAL, "cosmo_op_create: null argument");
   if (s->nv + s->zv > 24) return fail(COSMO_EUNSUP, "cosmo_op_create: more than 24 valence particles");
   if (s->dim >= (int64_t)0xffffffffLL) return fail(COSMO_EUNSUP, "cosmo_op_create: dimension beyond 2^32-1");
+  if (s->dim <= 0)
+    return fail(COSMO_EINVAL, "cosmo_op_create: empty basis (no many-body state has the system's M and parity)");
   auto *op = new cosmo_op_s();
   op->device = current_device();
   op->n = s->dim;

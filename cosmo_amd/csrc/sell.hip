@@ -49,7 +49,8 @@ namespace cosmo {
 #define SELL_MAXROWS 576  // rows of a row block: nine wavefronts, one lane per row
 #define SELL_PF 4         // entry slots per lane loaded one tile ahead (value panels)
 #define SELL_PFC 4        // the same for coded panels, whose slots take one register each
-#define SELL_MIN_N 65536  // smaller operators keep the row kernel
+#define SELL_MIN_N 262144  // automatic layout: smaller operators keep the row kernel
+#define SELL_AFTER 256     // automatic layout: built once the row kernel has made this many products
 #define SELL_DICT 256     // entries of a pair's value dictionary (entry 0 is 0.0)
 #define SELL_HASH 1024    // LDS hash slots that collect a pair's distinct values
 #define PM_PF 4           // slots per group in k_pm_apply (the slot arrays carry 4 groups of slack)
@@ -1489,7 +1490,16 @@ static int launch_sell_tuned(Sell &s, const T *X, T *Y, int m, hipStream_t st) {
 // 16-byte X rows up to 16 elements); COSMO_EUNSUP otherwise (the caller falls back).
 int sell_apply(Op &op, const void *X, void *Y, int m, int ld, int prec, hipStream_t st) {
   if (op.is_virtual || !op.csr.rowptr || op.layout == 1) return COSMO_EUNSUP;
-  if (op.layout == 0 && (g_sell_off || op.n < SELL_MIN_N)) return COSMO_EUNSUP;
+  // automatic layout: the tiled layout costs about 0.6 us per row to build (0.8 s at
+  // n = 1.4 M) and pays back about 2 ns per row and product at k = 8 from n of a few
+  // hundred thousand, some 400 products; below 262,144 rows it is no faster than the row
+  // kernel.  So it is built only for large operators that have already made SELL_AFTER
+  // products (one Davidson solve makes a few hundred): one-off solves stay on the row
+  // kernel, operators in long or repeated use switch.
+  if (op.layout == 0) {
+    if (g_sell_off || op.n < SELL_MIN_N) return COSMO_EUNSUP;
+    if (!op.sell && __atomic_add_fetch(&op.row_products, 1, __ATOMIC_RELAXED) < SELL_AFTER) return COSMO_EUNSUP;
+  }
   const bool f32 = prec == COSMO_PREC_FP32;
   if (!(f32 || prec == COSMO_PREC_FP64)) return COSMO_EUNSUP;
   if (f32 && !(ld == 4 || ld == 8 || ld == 16)) return COSMO_EUNSUP;

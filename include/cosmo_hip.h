@@ -190,10 +190,12 @@ int cosmo_op_write_csb(cosmo_op op, const char *path, int block);
  * for n <= limit (include/xsm/DEXHVcsb.cxx:72-108), computed on the GPU: the rows are
  * scattered into a dense FP64 matrix in HBM and solved by rocSOLVER (dsyevd). */
 int cosmo_op_dense_eigen(cosmo_op op, double *evals, double *evecs);
-/* Kernel of the stored FP32 / FP64 product: 0 automatic (the tiled layout for n >= 65536,
- * the row kernel below), 1 the row kernel, 2 the tiled layout at any size.  The tiled
- * layout (row blocks of whole neutron configurations, X tiles staged in LDS) is built on
- * the first product that uses it and kept with the operator.  No reference counterpart:
+/* Kernel of the stored FP32 / FP64 product: 0 automatic (the row kernel, then, for n >=
+ * 262144, the tiled layout from the operator's 257th product on: it takes about 0.6 us per
+ * row to build and repays that over some 400 products), 1 the row kernel, 2 the tiled
+ * layout at any size from the next product on.  The tiled layout (row blocks of whole
+ * neutron configurations, X tiles staged in LDS, proton-spectator tiles, value
+ * dictionaries) is built on the first product that uses it and kept with the operator.  No reference counterpart:
  * a kernel choice of this build; every choice sums each row in ascending column order
  * within its kernel's partial sums and is deterministic. */
 int cosmo_op_set_layout(cosmo_op op, int layout);

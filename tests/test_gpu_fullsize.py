@@ -32,6 +32,7 @@ def full(gpu, tmp_path_factory):
     assert s.dim == N_FULL
     print("fp 4p4n: building H", flush=True)
     op = gpu.Operator.build(s)
+    op.set_layout(2)  # the tiled layout from the first product (the library builds it after 256)
     o = oracle.System(xml)
     b = o.basis()
     yield s, op, o, b

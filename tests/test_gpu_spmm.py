@@ -84,7 +84,7 @@ def test_apply_device_torch(gpu, golden):
 @pytest.mark.parametrize("name", ["sd2p2n_s11", "sd3p1n_m1_s7", "fp2p2n_s5", "fp3p2n_m1_s3", "sd4p4n_s0"])
 def test_tiled_layout_vs_oracle_and_rows(gpu, golden, name):
     """The tiled stored product (row blocks of whole neutron configurations, X tiles in
-    LDS, used by default from n = 65536) forced on the golden systems: FP64 within 1e-12
+    LDS, used by default from n = 262144 after 256 products) forced on the golden systems: FP64 within 1e-12
     and FP32 within 2e-5 of the oracle's CSB product for every row pitch it serves, the
     same bits on a repeated call, and the row kernel's result to rounding."""
     g = golden(name)
@@ -121,3 +121,35 @@ def test_tiled_layout_davidson(gpu, golden):
         eb, _, rb, _, sb = gpu.davidson(b, 10, want_vectors=False, critr=1e-6, lim=40, precision=prec)
         assert sa["converged"] and sb["converged"]
         assert np.abs(ea - eb).max() < 1e-9 and ra.max() <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["psd2p2n_hw2", "fp3p2n_m1_rej", "fp2p2n_s5_jj", "rows:fp2p2n_s5"])
+def test_tiled_layout_without_the_proton_spectator_split(gpu, golden, name):
+    """The tiled product forced where its proton-spectator split does not apply (a truncated
+    basis; an operator uploaded from upper rows, which carries no basis structure: blocks of
+    576 rows) or where added JJ terms widen each pair's set of values: FP64 within 1e-12 and
+    FP32 within 2e-5 of the CSB product of the oracle's rows (of the rows the reference's
+    XHH+JJ wrote, for the JJ system) at every row pitch it serves."""
+    from conftest import TRUNC_CASES, trunc_case
+    from_rows = name.startswith("rows:")
+    name = name.split(":")[-1]
+    xml = trunc_case(name)[0] if name in TRUNC_CASES else golden(name).xml
+    s = gpu.System.from_xml(xml)
+    if name.endswith("_jj"):
+        a = golden(name).arrays
+        rp, c, v = a["hh_rowptr"], a["hh_cols"], a["hh_vals"]
+    else:
+        rp, c, v = oracle.System(xml).h_upper(basis=s.basis(), drop_small=True)
+    n = len(rp) - 1
+    if from_rows:
+        op = gpu.Operator.from_upper(n, np.asarray(rp, np.int64), np.asarray(c, np.int64), np.asarray(v))
+    else:
+        op = gpu.Operator.build(s)
+    op.set_layout(2)
+    assert op.n == n
+    for m in (1, 3, 4, 8, 16):
+        X = np.random.default_rng(10 + m).standard_normal((n, m))
+        ref = oracle.csb_times(n, rp, c, v, X)
+        scale = np.abs(ref).max()
+        assert np.abs(op.apply(X, gpu.PREC_FP64) - ref).max() <= 1e-12 * scale, m
+        assert np.abs(op.apply(X, gpu.PREC_FP32) - ref).max() <= 2e-5 * scale, m

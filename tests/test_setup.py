@@ -226,3 +226,16 @@ def test_truncation_input_errors(tmp_path):
         xml = synth.write_system(str(tmp_path), "psd", "b%d" % i, 2, 2, 0, "+", 1, system_children=kids)
         with pytest.raises(cosmo_amd.CosmoError):
             cosmo_amd.System.from_xml(xml)
+
+
+def test_empty_basis_is_refused_before_any_device_work(tmp_path):
+    """An odd number of valence nucleons at 2M = 0 has no M-scheme state: the basis is
+    empty and building an operator on it fails with COSMO_EINVAL (a clear message, before
+    any device call)."""
+    import cosmo_amd as ca
+    from cosmo_amd import synth
+    s = ca.System.from_xml(synth.write_system(str(tmp_path), "sd", "odd", 2, 3, 0, "+", 1))
+    assert s.dim == 0
+    with pytest.raises(ca.CosmoError) as e:
+        ca.Operator.build(s)
+    assert e.value.code == ca.EINVAL and "empty basis" in str(e.value)
