@@ -54,6 +54,7 @@ namespace cosmo {
 #define SELL_DICT 256     // entries of a pair's value dictionary (entry 0 is 0.0)
 #define SELL_HASH 1024    // LDS hash slots that collect a pair's distinct values
 #define PM_PF 4           // slots per group in k_pm_apply (the slot arrays carry 4 groups of slack)
+static_assert(SELL_PFC >= SELL_PF, "panel slot arrays carry SELL_PFC * 64 slots of slack for both kinds");
 
 // one set of panels over the row blocks: per row block a subset of the column tiles it
 // meets (ascending), per (row block, tile, 64-row slice) an ELL panel
@@ -512,10 +513,10 @@ __device__ __forceinline__ void sell_apply_body(const SellView &A, const T *__re
   const int ns = (S + 63) >> 6;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x, wc = min(w, ns - 1);
   const bool wave_on = w < ns;
-  // LDS: two tile buffers of nthr + 1 rows (row nthr stays zero: the padding slots' row;
+  // LDS: NB tile buffers of nthr + 1 rows (row nthr stays zero: the padding slots' row;
   // the direct loads fill rows [0, nthr)), then the block's tile table (first X row,
   // 16-byte pieces, dictionary offset and length) and its slice offsets (64-slot units from
-  // the block's first slot), then (coded) two dictionary buffers
+  // the block's first slot), then (coded) NB dictionary buffers
   const int zrow = A.nthr;
   V *buf0 = reinterpret_cast<V *>(lds_raw), *buf1 = buf0 + (size_t)(NB - 1) * (zrow + 1) * VPR;
   uint32_t *mt_c0 = reinterpret_cast<uint32_t *>(buf0 + (size_t)NB * (zrow + 1) * VPR);
@@ -799,7 +800,7 @@ __global__ __launch_bounds__(SELL_MAXROWS) void k_pm_apply(PmView P, const T *__
   T ha[PF], hb[PF];
   auto load = [&](int j0, uint32_t (&c)[PF], T (&h)[PF]) {
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {  // the slot arrays carry PF * 64 slots of slack
+    for (int j = 0; j < PF; ++j) {  // the slot arrays carry 4 * PF * 64 slots of slack
       c[j] = P.cols[base + (int64_t)(j0 + j) * 64];
       h[j] = vals[base + (int64_t)(j0 + j) * 64];
     }
