@@ -1,18 +1,22 @@
 // batch.hip -- many independent solves on one device: a batch call and a serving loop.
 //
-// A batch splits its solves by size.  The small ones (dav_wg.h bounds) run to completion in
-// one kernel launch, one workgroup each, once worker threads have built their operators;
-// the others own their operator and Davidson blocks and run on their own HIP streams from
-// the worker threads, so solves of uneven size overlap on the device.  All kernels are
-// deterministic (no atomics in H.X, reductions in fixed order), so a solve gives the same
-// bits alone and in a mix on either path.  The server admits queued requests in arrival order as
+// A batch splits its solves by size.  Those up to the single-launch bound (and within the
+// dav_wg.h bounds) run to completion in ONE persistent kernel launch once worker threads have
+// built their operators, each solve by a team of workgroups sized to its dimension (one
+// workgroup per 2048 rows); the others own their operator and Davidson blocks and run on
+// their own HIP streams from the worker threads, beside the launch.  All kernels are
+// deterministic (no atomics in H.X, reductions in fixed order, team sizes fixed by the
+// system), so a solve gives the same bits alone and in a mix on either path.  The server
+// admits queued requests in arrival order as
 // soon as a worker is free and the request's exact device footprint (operator from the
 // builder's count pass + Davidson blocks) fits the remaining HBM budget.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -75,8 +79,9 @@ static int solve_one(System *s, int flags, const cosmo_dav_params &p0, int nume,
 
 
 // ---- single-launch path ---------------------------------------------------------------------
-// Solves inside the bounds of dav_wg.h, on stored operators already built on the device: one
-// workgroup per solve, all in one launch on the calling thread's stream.
+// Solves inside the bounds of dav_wg.h, on stored operators already built on the device: all
+// of them in one persistent launch on the calling thread's stream, each by a team of
+// workgroups sized to its dimension.
 struct SmallJob {
   cosmo_op op;
   cosmo_dav_params p;  // as batch_params gives it
@@ -91,29 +96,74 @@ static bool fits_single_launch(const cosmo_dav_params &p, int64_t n, int flags, 
          !((flags & COSMO_OP_NO_FP32) && p.precision == COSMO_PREC_FP32) && p.ninit == 0;
 }
 
+// Rows per workgroup of a team (COSMO_DWG_ROWS overrides): a team of G workgroups pays a
+// barrier per exchange, about ten per iteration, against rows / G of streaming each.
+static int64_t dwg_rows() {
+  static const int64_t v = [] {
+    const char *e = getenv("COSMO_DWG_ROWS");
+    const long long x = e ? atoll(e) : 0;
+    return (int64_t)(x > 0 ? x : 2048);
+  }();
+  return v;
+}
+
+// Team size and lanes per row of H.X of a solve: functions of the system (and of a
+// device too small for the team), so a solve gives the same bits alone and in any mix.
+static void team_shape(int64_t n, int64_t nnz, int cap, int *G, int *lanes) {
+  const int64_t rows = dwg_rows();
+  int64_t g = (n + rows - 1) / rows;
+  g = std::max<int64_t>(1, std::min<int64_t>(g, std::min(DWG_GMAX, cap)));
+  *G = (int)g;
+  const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
+  int L = 1;
+  while (L < 64 && L * 16 <= avg) L *= 2;  // about 8 to 16 entries per lane
+  *lanes = L;
+}
+
 static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
   const int ns = (int)jobs.size();
   if (!ns) return COSMO_OK;
   hipStream_t st = lib_stream();
+  int dev = 0, ncu = 0, per_cu = 0, wclk_khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dav_wg, DWG_THREADS, 0));
+  const int cap = std::max(1, per_cu * ncu);  // workgroups resident at once
+  // team shapes; seats in order of decreasing team size, then dimension
+  std::vector<int> G(ns), L(ns), ord(ns);
+  for (int j = 0; j < ns; ++j) {
+    team_shape(jobs[j].op->n, jobs[j].op->csr.nnz, cap, &G[j], &L[j]);
+    ord[j] = j;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    if (G[a] != G[b]) return G[a] > G[b];
+    return jobs[a].op->n > jobs[b].op->n;
+  });
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   std::vector<size_t> off(ns + 1, 0);
   for (int j = 0; j < ns; ++j) {
-    const int64_t n = jobs[j].op->n;
-    off[j + 1] = off[j] + up(sizeof(double) * (size_t)n * (3 * (size_t)jobs[j].p.lim + 2 * DWG_MB));
+    const size_t n = (size_t)jobs[j].op->n;
+    off[j + 1] = off[j] + up(sizeof(double) * (n * (3 * (size_t)jobs[j].p.lim + 2 * DWG_MB) +
+                                               2 * (size_t)G[j] * DWG_PART));
   }
   const size_t obytes = sizeof(double) * 3 * DWG_MB * (size_t)ns, ibytes = sizeof(int) * 6 * (size_t)ns;
+  const size_t cbytes = sizeof(unsigned) * ((size_t)ns + 1);
   char *arena = nullptr, *outs = nullptr;
   DwgSolve *ddesc = nullptr;
   HIPCHK(dmalloc(&arena, off[ns], st));
-  HIPCHK(dmalloc(&outs, up(obytes) + ibytes, st));
+  HIPCHK(dmalloc(&outs, up(obytes) + up(ibytes) + cbytes, st));
   HIPCHK(dmalloc(&ddesc, sizeof(DwgSolve) * ns, st));
   std::vector<DwgSolve> desc(ns);
   double *dout = (double *)outs;
   int *dinfo = (int *)(outs + up(obytes));
-  for (int j = 0; j < ns; ++j) {
+  unsigned *dbar = (unsigned *)(outs + up(obytes) + up(ibytes));  // ns team counters, then the seat counter
+  int nseats = 0;
+  for (int q = 0; q < ns; ++q) {
+    const int j = ord[q];
     const Op &o = *jobs[j].op;
     const cosmo_dav_params &p = jobs[j].p;
-    DwgSolve &d = desc[j];
+    DwgSolve &d = desc[q];
     const int64_t n = o.n;
     double *w = (double *)(arena + off[j]);
     d.n = n;
@@ -126,7 +176,13 @@ static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
     d.AB = w + n * p.lim;
     d.R = w + 2 * n * p.lim;
     d.X = w + 3 * n * p.lim;
-    d.Y = d.X + n * DWG_MB;
+    d.part = d.X + 2 * n * DWG_MB;
+    d.bar = dbar + q;
+    d.claim = dbar + ns;
+    d.G = G[j];
+    d.lanes = L[j];
+    d.seat0 = nseats;
+    nseats += G[j];
     d.nume = p.nume;
     d.lim = p.lim;
     d.mblock = p.mblock;
@@ -138,6 +194,7 @@ static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
     d.info = dinfo + 6 * j;
   }
   HIPCHK(hipMemcpyAsync(ddesc, desc.data(), sizeof(DwgSolve) * ns, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(dbar, 0, cbytes, st));
   DwgParams prm;
   prm.crite = p0.crite;
   prm.critc = p0.critc;
@@ -145,8 +202,11 @@ static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
   prm.switch_tol = p0.mixed_switch > 0 ? p0.mixed_switch : 1e-5;
   prm.strict = p0.strict;
   prm.mixed = p0.precision == COSMO_PREC_FP32;
+  prm.spin_ticks = (uint64_t)std::max(wclk_khz, 1) * 1000ull * 10ull;  // 10 s
+  const int grid = std::min(nseats, cap);
   auto t0 = std::chrono::steady_clock::now();
-  hipLaunchKernelGGL(k_dav_wg, dim3(ns), dim3(DWG_THREADS), 0, st, (const DwgSolve *)ddesc, prm);
+  hipLaunchKernelGGL(k_dav_wg, dim3(grid), dim3(DWG_THREADS), 0, st, (const DwgSolve *)ddesc, ns, nseats,
+                     dbar + ns, prm);
   HIPCHK(hipGetLastError());
   std::vector<double> hout(3 * DWG_MB * (size_t)ns);
   std::vector<int> hinfo(6 * (size_t)ns);
@@ -172,7 +232,9 @@ static int run_small(std::vector<SmallJob> &jobs, const cosmo_dav_params &p0) {
     jb.st->restarts = inf[3];
     jb.st->ierr = inf[4];
     jb.st->ms_total = ms;
-    jb.rc = inf[5] ? COSMO_OK : COSMO_ECONV;
+    if (inf[4] & DWG_ABORTED)
+      jb.rc = fail(COSMO_EHIP, "cosmo_batch_solve: a team barrier of the single launch timed out");
+    else jb.rc = inf[5] ? COSMO_OK : COSMO_ECONV;
   }
   return COSMO_OK;
 }
@@ -185,7 +247,10 @@ struct cosmo_batch_s {
   std::vector<cosmo_system> sys;
   int flags = 0;
   int device = 0;
-  int64_t single_nmax = 2048;  // dimension bound of the single-launch path (0: off)
+  // dimension bound of the single-launch path (0: off).  On the 256-solve sd batch (n up to
+  // 93,710) bounds of 2048 / 4096 / 8192 / 16384 / 32768 / all took 1.28 / 1.23 / 1.24 /
+  // 1.13 / 1.17 / 1.29 s: the largest solves run faster host-driven beside the launch.
+  int64_t single_nmax = 16384;
 };
 
 struct Request {

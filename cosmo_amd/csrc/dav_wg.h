@@ -1,6 +1,11 @@
-// dav_wg.h -- block Davidson for one small system per workgroup: every batched solve whose
-// dimension, basis limit and block fit the bounds below runs to completion inside a single
-// launch, one workgroup per solve.
+// dav_wg.h -- block Davidson for batched solves, all of them in one persistent launch.
+//
+// Every solve is run to completion by a team of G workgroups (G = 1 for the small ones).
+// Workgroups take seats in the launch's seat order (solves by decreasing team size), each
+// seat being one rank of one solve, and take the next seat when their solve ends.  Seats
+// are handed out in order, so every solve whose seats have all been taken has all of its
+// workgroups running: a team waits for its last members only until earlier solves end,
+// and the launch cannot deadlock while the largest team fits the device at once.
 //
 // The iteration restates the host driver (cosmo_davidson in davidson.hip, itself dvdrvr2 of
 // include/kdav/fsdavidson.cxx:615-775) step for step: unit start vectors at the lowest
@@ -9,24 +14,33 @@
 // (Add_Vectors_To_Basis :426-516, floor 1e-13), blocked Gram-Schmidt passes with a
 // pivot-dropping Cholesky QR, restarts by collapse (:99-121, :517-539), the switch from the
 // FP32 product to FP64 refinement, and the strict re-opening of pairs whose residual
-// drifted.  Every n-long operation is spread over the workgroup; the (kpass x kpass)
-// Rayleigh-Ritz problem is solved in LDS by parallel cyclic Jacobi, warm-started from the
-// previous Ritz vectors (the basis only grows between restarts, so the previous vectors
-// nearly diagonalise the new matrix).  Decisions are taken by thread 0 and published
-// through LDS, so control flow is uniform over the workgroup and every barrier is reached
-// by all of its threads.  Basis, image and residual blocks are FP64 and column-major
-// (ld n); H.X runs on a row-major copy of the block (one cache line per stored entry).
+// drifted.  Basis, image and residual blocks are FP64 and column-major (ld n).
 //
-// Only plain HIP constructs are used (threadIdx, blockDim, __syncthreads, __shared__), and
-// every parallel loop is a stride loop over its work items, with no intra-wave shuffles or
-// atomics: the kernel's result does not depend on the workgroup size.
+// Rows are split among the team in contiguous ranges, and every n-long step is row-local
+// except three, which exchange data through global memory between team barriers: H.X (the
+// block to multiply is staged row-major, each workgroup writing its own rows, and gathered
+// by all), the Gram blocks (one partial block per workgroup, summed by every workgroup in
+// rank order, so that all hold the same bits) and the choice of start vectors (each
+// workgroup's lowest diagonal elements, merged by every workgroup).  Each workgroup then
+// solves the same (kpass x kpass) Rayleigh-Ritz problem in its own LDS by parallel cyclic
+// Jacobi, warm-started from the previous Ritz vectors, and takes the same decisions (thread
+// 0, published through LDS), so control flow is uniform over the team and every barrier is
+// reached by all of its threads.  Staging and partial buffers alternate between two copies
+// with the barrier count, so one barrier per exchange suffices.  A team barrier that waits
+// longer than the launch's limit marks the solve failed instead of waiting on.
+//
+// Results depend on the team size and on the lanes per row of H.X (fixed functions of the
+// system), never on the other solves of the launch or on timing.
 
 #include "jacobi.h"
 
 #define DWG_THREADS 512
-#define DWG_LIM 48     // basis limit (lim) bound
-#define DWG_MB 16      // nume and mblock bound
-#define DWG_NMAX 32768 // dimension bound
+#define DWG_LIM 48                    // basis limit (lim) bound
+#define DWG_MB 16                     // nume and mblock bound
+#define DWG_GMAX 128                  // team size bound
+#define DWG_NMAX ((int64_t)1 << 30)   // dimension bound (row numbers of the start vectors are int)
+#define DWG_PART (DWG_LIM * DWG_MB)   // doubles of one workgroup's partial block
+#define DWG_ABORTED 2048              // ierr bit: a team barrier timed out
 
 struct DwgSolve {
   int64_t n;
@@ -36,7 +50,13 @@ struct DwgSolve {
   const float *v32;
   const double *diag;
   double *B, *AB, *R;  // n x lim each, column-major
-  double *X, *Y;       // n x DWG_MB each, row-major staging of H.X
+  double *X;           // two staging buffers of n x DWG_MB doubles (the block to multiply)
+  double *part;        // two buffers of G x DWG_PART doubles (the team's partial blocks)
+  unsigned *bar;       // team barrier counter, zero at launch
+  unsigned *claim;     // the launch's seat counter
+  int G;               // workgroups in the team
+  int lanes;           // lanes per row in H.X: a power of two, 1..64
+  int seat0;           // first seat of the solve; seats ascend with the array index
   int nume, lim, mblock, maxiter;
   double critr;
   double *evals, *dE, *resid;  // nume each
@@ -46,14 +66,15 @@ struct DwgSolve {
 struct DwgParams {
   double crite, critc, ortho, switch_tol;
   int strict, mixed;
+  uint64_t spin_ticks;  // longest wait at a team barrier once the team is seated, wall-clock ticks
 };
 
 struct DwgShared {
   double S[DWG_LIM * DWG_LIM];       // Rayleigh matrix B^T H B (full, row-major)
   double A[DWG_LIM * DWG_LIM];       // Jacobi matrix; R factor of the Cholesky QR
   double V[DWG_LIM * DWG_LIM];       // Ritz vectors in the basis: V[r * DWG_LIM + e]
-  double W2[2 * DWG_LIM * DWG_LIM];  // Jacobi second buffers / Gram partial sums / T = R^-1
-  double G[DWG_LIM * DWG_LIM];       // Gram results, row-major (P x M)
+  double W2[2 * DWG_LIM * DWG_LIM];  // Jacobi second buffers / T = R^-1
+  double G[DWG_LIM * DWG_MB];        // Gram results, row-major (P x M)
   double Wt[DWG_LIM * DWG_LIM];      // overlaps B^T V of an orthogonalisation pass
   double C1[DWG_LIM * DWG_MB], C2[DWG_LIM * DWG_MB];
   double eigs[DWG_LIM], old[DWG_MB], rnorm[DWG_MB], lam[DWG_MB], base[DWG_LIM];
@@ -62,26 +83,92 @@ struct DwgShared {
   int mini[DWG_THREADS];
   int partner[DWG_LIM], pairidx[DWG_LIM], order[DWG_LIM], keep[DWG_LIM];
   int icv[DWG_MB], incv[DWG_MB], add[DWG_MB], chosen[DWG_MB];
+  DwgSolve sol;   // the solve this workgroup runs
+  DwgParams prm;
+  // team: own rows [r0, r1), rank, barriers passed, timed out
+  int64_t r0, r1;
+  int rank, phase, aborted, seat;
   // control state, written by thread 0 between barriers
   int kpass, nncv, kv, fp64, first, omin_to, nloops, restarts, ierr, verify;
   int done, flag, kept, restart, ncand, nadded, matvecs, matvecs64, rot;
   double dprev, dctl;
 };
 
-// ---- parallel building blocks (every thread of the workgroup calls them) ----------------
+// ---- team building blocks (every thread of every workgroup of the team calls them) -------
 
-// G[a * M + b] = sum_r X[r + a n] Y[r + b n], a < P <= DWG_LIM, b < M <= DWG_MB.  Threads own
-// 4 x 4 output blocks over interleaved row groups; the groups' sums are added in group order.
-__device__ void dwg_gram(DwgShared &sh, int64_t n, const double *X, int P, const double *Y, int M) {
-  const int tid = threadIdx.x;
-  if (P <= 0 || M <= 0) return;  // uniform: every thread sees the same P, M
+// Team barrier: returns once all G workgroups have arrived (the writes before it visible to
+// the whole team), or with sh.aborted set after waiting prm.spin_ticks from the moment every
+// rank of the team held its seat (until then teammates may still be finishing earlier
+// solves); later calls of an aborted workgroup wait no more.
+__device__ void dwg_sync(DwgShared &sh, const DwgSolve &s, const DwgParams &prm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s.G > 1 && !sh.aborted) {
+      __threadfence();
+      const unsigned target = (unsigned)s.G * (unsigned)(sh.phase + 1);
+      const unsigned seated = (unsigned)(s.seat0 + s.G);
+      atomicAdd(s.bar, 1u);
+      bool all_seated = false;
+      uint64_t t0 = 0;
+      while (__hip_atomic_load(s.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (!all_seated) {
+          all_seated = __hip_atomic_load(s.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seated;
+          t0 = wall_clock64();
+        } else if (wall_clock64() - t0 > prm.spin_ticks) {
+          sh.aborted = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __threadfence();
+    }
+    ++sh.phase;
+  }
+  __syncthreads();
+}
+
+// Sum of 16 values over the wave's 64 lanes by recursive halving: afterwards lane l with
+// l % 4 == 0 holds in v[0] the total of value 8 b5 + 4 b4 + 2 b3 + b2 (bits of l).
+__device__ inline void dwg_wave_sum16(double (&v)[16], int lane) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool hi = lane & 32;
+    const double snd = hi ? v[i] : v[i + 8], kp = hi ? v[i + 8] : v[i];
+    v[i] = kp + __shfl_xor(snd, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool hi = lane & 16;
+    const double snd = hi ? v[i] : v[i + 4], kp = hi ? v[i + 4] : v[i];
+    v[i] = kp + __shfl_xor(snd, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const bool hi = lane & 8;
+    const double snd = hi ? v[i] : v[i + 2], kp = hi ? v[i + 2] : v[i];
+    v[i] = kp + __shfl_xor(snd, 8);
+  }
+  {
+    const bool hi = lane & 4;
+    const double snd = hi ? v[0] : v[1], kp = hi ? v[1] : v[0];
+    v[0] = kp + __shfl_xor(snd, 4);
+  }
+  v[0] += __shfl_xor(v[0], 2);
+  v[0] += __shfl_xor(v[0], 1);
+}
+
+// sh.G[a * M + b] = sum over all n rows of X[r + a n] Y[r + b n], a < P <= DWG_LIM,
+// b < M <= DWG_MB.  A wave owns a 4 x 4 output block over the workgroup's rows (lanes on
+// consecutive rows); the team's partial blocks are added in rank order.
+__device__ __attribute__((noinline)) void dwg_gram(DwgShared &sh, const DwgSolve &s, const DwgParams &prm, const double *X, int P,
+                         const double *Y, int M) {
+  if (P <= 0 || M <= 0) return;  // uniform: every rank sees the same P, M
   const int nbA = (P + 3) / 4, nbB = (M + 3) / 4, nblk = nbA * nbB;
-  int RG = 256 / nblk;
-  if (RG < 1) RG = 1;
-  const int T = nblk * RG;
-  double *part = sh.W2;
-  for (int t = tid; t < T; t += blockDim.x) {
-    const int blk = t % nblk, g = t / nblk;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int64_t n = s.n, r0 = sh.r0, r1 = sh.r1;
+  const int par = sh.phase & 1;
+  double *dst = s.G > 1 ? s.part + ((int64_t)par * s.G + sh.rank) * DWG_PART : sh.G;
+  for (int blk = wave; blk < nblk; blk += nwave) {
     const int a0 = (blk / nbB) * 4, b0 = (blk % nbB) * 4;
     const double *xa[4], *yb[4];
     for (int i = 0; i < 4; ++i) {
@@ -90,7 +177,8 @@ __device__ void dwg_gram(DwgShared &sh, int64_t n, const double *X, int P, const
     }
     double acc[16];
     for (int i = 0; i < 16; ++i) acc[i] = 0.0;
-    for (int64_t r = g; r < n; r += RG) {
+#pragma unroll 2
+    for (int64_t r = r0 + lane; r < r1; r += 64) {
       double x[4], y[4];
       for (int i = 0; i < 4; ++i) {
         x[i] = xa[i][r];
@@ -99,28 +187,50 @@ __device__ void dwg_gram(DwgShared &sh, int64_t n, const double *X, int P, const
       for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) acc[i * 4 + j] += x[i] * y[j];
     }
-    for (int i = 0; i < 16; ++i) part[(int64_t)t * 16 + i] = acc[i];
+    dwg_wave_sum16(acc, lane);
+    if ((lane & 3) == 0) {
+      const int ij = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+      const int a = a0 + ij / 4, b = b0 + ij % 4;
+      if (a < P && b < M) dst[a * M + b] = acc[0];
+    }
   }
-  __syncthreads();
-  for (int o = tid; o < P * M; o += blockDim.x) {
-    const int a = o / M, b = o % M;
-    const int blk = (a / 4) * nbB + b / 4, ij = (a % 4) * 4 + b % 4;
-    double s = 0.0;
-    for (int g = 0; g < RG; ++g) s += part[(int64_t)(g * nblk + blk) * 16 + ij];
-    sh.G[o] = s;
+  dwg_sync(sh, s, prm);
+  if (s.G > 1) {
+    const double *src = s.part + (int64_t)par * s.G * DWG_PART;
+    for (int o = threadIdx.x; o < P * M; o += blockDim.x) {
+      double t = 0.0;
+      for (int g = 0; g < s.G; ++g) t += src[(int64_t)g * DWG_PART + o];
+      sh.G[o] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
-// Y[:, j] = X1 C1[:, j] (+ X2 C2[:, j]) for j < q <= DWG_MB, k < p; C row-major p x DWG_MB in
-// LDS.  Row by row, all reads of a row before its writes, so Y may be X1 or X2 when q <= p
-// covers only columns already read (callers use distinct arrays or one chunk).
-__device__ void dwg_combine(const DwgShared &sh, int64_t n, const double *X1, const double *X2, int p, int q,
-                            double *Yc) {
-  for (int64_t r = threadIdx.x; r < n; r += blockDim.x) {
+// own rows of Y[:, j] = X1 C1[:, j] (+ X2 C2[:, j]) for j < q <= DWG_MB, k < p; C row-major
+// p x DWG_MB in LDS.  Row by row, all reads of a row before its writes, so Y may be X1 or X2
+// when q <= p covers only columns already read (callers use distinct arrays or one chunk).
+__device__ __attribute__((noinline)) void dwg_combine(const DwgShared &sh, const DwgSolve &s, const double *X1, const double *X2, int p,
+                            int q, double *Yc) {
+  const int64_t n = s.n, r1 = sh.r1;
+#pragma unroll 1
+  for (int64_t r = sh.r0 + threadIdx.x; r < r1; r += blockDim.x) {
     double acc[DWG_MB];
     for (int j = 0; j < DWG_MB; ++j) acc[j] = 0.0;
-    for (int k = 0; k < p; ++k) {
+    int k = 0;
+#pragma unroll 1
+    for (; k + 4 <= p; k += 4) {  // four columns' loads in flight
+      double a[4], b[4];
+      for (int u = 0; u < 4; ++u) a[u] = X1[r + (int64_t)(k + u) * n];
+      if (X2)
+        for (int u = 0; u < 4; ++u) b[u] = X2[r + (int64_t)(k + u) * n];
+      for (int u = 0; u < 4; ++u) {
+        for (int j = 0; j < DWG_MB; ++j) acc[j] += a[u] * sh.C1[(k + u) * DWG_MB + j];
+        if (X2)
+          for (int j = 0; j < DWG_MB; ++j) acc[j] += b[u] * sh.C2[(k + u) * DWG_MB + j];
+      }
+    }
+#pragma unroll 1
+    for (; k < p; ++k) {
       const double a = X1[r + (int64_t)k * n];
       for (int j = 0; j < DWG_MB; ++j) acc[j] += a * sh.C1[k * DWG_MB + j];
       if (X2) {
@@ -134,74 +244,101 @@ __device__ void dwg_combine(const DwgShared &sh, int64_t n, const double *X1, co
   __syncthreads();
 }
 
-// AB[:, c0 .. c0+m) = H B[:, c0 .. c0+m) through the row-major staging, in chunks of DWG_MB
-__device__ void dwg_apply(DwgShared &sh, const DwgSolve &s, int c0, int m, bool fp64) {
-  const int64_t n = s.n;
-  for (int j0 = 0; j0 < m; j0 += DWG_MB) {
-    const int mm = min(DWG_MB, m - j0);
-    for (int64_t t = threadIdx.x; t < n * mm; t += blockDim.x) {
-      const int64_t r = t / mm;
-      const int j = (int)(t % mm);
-      s.X[r * DWG_MB + j] = s.B[r + (int64_t)(c0 + j0 + j) * n];
+// own rows of dst[:, 0..q) = src[:, 0..q)
+__device__ __attribute__((noinline)) void dwg_copy_cols(const DwgShared &sh, const DwgSolve &s, double *dst, const double *src, int q) {
+  const int64_t n = s.n, r0 = sh.r0, m = sh.r1 - sh.r0;
+  for (int64_t t = threadIdx.x; t < m * q; t += blockDim.x) {
+    const int64_t o = (t / m) * n + r0 + t % m;
+    dst[o] = src[o];
+  }
+  __syncthreads();
+}
+
+// own rows of the staging buffer: xs[r * PW + j] = B[r + (c + j) n] for j < mm, 0 up to PW
+template <typename T, int PW>
+__device__ __attribute__((noinline)) void dwg_stage(const DwgShared &sh, const DwgSolve &s, T *xs, int c, int mm) {
+  const int64_t n = s.n, r0 = sh.r0, m = sh.r1 - sh.r0;
+  for (int64_t t = threadIdx.x; t < m * PW; t += blockDim.x) {
+    const int64_t r = r0 + t / PW;
+    const int j = (int)(t % PW);
+    xs[r * PW + j] = j < mm ? (T)s.B[r + (int64_t)(c + j) * n] : (T)0;
+  }
+}
+
+// own rows of AB[:, c .. c+mm) = H (staged block): s.lanes lanes per row, each taking every
+// lanes-th entry, then summed across the lanes by butterfly (all lanes end with the same
+// bits); lane j % lanes writes column j
+template <typename T, int PW>
+__device__ __attribute__((noinline)) void dwg_hx_rows(const DwgShared &sh, const DwgSolve &s, const T *xs, const T *vals, int c, int mm) {
+  const int L = s.lanes;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int rpw = 64 / L, sub = lane & (L - 1);
+  const int64_t n = s.n, r1 = sh.r1;
+#pragma unroll 1
+  for (int64_t rb = sh.r0 + (int64_t)wave * rpw; rb < r1; rb += (int64_t)nwave * rpw) {
+    const int64_t r = rb + lane / L;
+    int64_t e0 = 0, e1 = 0;
+    if (r < r1) {
+      e0 = s.rowptr[r];
+      e1 = s.rowptr[r + 1];
     }
-    __syncthreads();
-    for (int64_t r = threadIdx.x; r < n; r += blockDim.x) {
-      const int64_t e0 = s.rowptr[r], e1 = s.rowptr[r + 1];
-      if (fp64) {
-        double acc[DWG_MB];
-        for (int j = 0; j < DWG_MB; ++j) acc[j] = 0.0;
-        int64_t e = e0;
-        for (; e + 4 <= e1; e += 4) {  // four entries per step: independent loads in flight
-          double h[4];
-          const double *x[4];
-          for (int u = 0; u < 4; ++u) {
-            h[u] = s.v64[e + u];
-            x[u] = s.X + (int64_t)s.cols[e + u] * DWG_MB;
-          }
-          for (int u = 0; u < 4; ++u)
-            for (int j = 0; j < DWG_MB; ++j)
-              if (j < mm) acc[j] += h[u] * x[u][j];
-        }
-        for (; e < e1; ++e) {
-          const double h = s.v64[e];
-          const double *x = s.X + (int64_t)s.cols[e] * DWG_MB;
-          for (int j = 0; j < DWG_MB; ++j)
-            if (j < mm) acc[j] += h * x[j];
-        }
-        for (int j = 0; j < DWG_MB; ++j)
-          if (j < mm) s.Y[r * DWG_MB + j] = acc[j];
-      } else {
-        float acc[DWG_MB];
-        for (int j = 0; j < DWG_MB; ++j) acc[j] = 0.0f;
-        int64_t e = e0;
-        for (; e + 4 <= e1; e += 4) {
-          float h[4];
-          const double *x[4];
-          for (int u = 0; u < 4; ++u) {
-            h[u] = s.v32[e + u];
-            x[u] = s.X + (int64_t)s.cols[e + u] * DWG_MB;
-          }
-          for (int u = 0; u < 4; ++u)
-            for (int j = 0; j < DWG_MB; ++j)
-              if (j < mm) acc[j] += h[u] * (float)x[u][j];
-        }
-        for (; e < e1; ++e) {
-          const float h = s.v32[e];
-          const double *x = s.X + (int64_t)s.cols[e] * DWG_MB;
-          for (int j = 0; j < DWG_MB; ++j)
-            if (j < mm) acc[j] += h * (float)x[j];
-        }
-        for (int j = 0; j < DWG_MB; ++j)
-          if (j < mm) s.Y[r * DWG_MB + j] = (double)acc[j];
+    T acc[PW];
+    for (int j = 0; j < PW; ++j) acc[j] = (T)0;
+    // U entries per step, all their loads issued before the first product: the gathers
+    // are latency-bound with one workgroup per CU
+    constexpr int U = PW * sizeof(T) <= 16 ? 8 : (PW * sizeof(T) <= 32 ? 4 : 2);
+    int64_t e = e0 + sub;
+#pragma unroll 1
+    for (; e + (U - 1) * L < e1; e += U * L) {
+      T h[U], x[U][PW];
+      uint32_t c[U];
+      for (int u = 0; u < U; ++u) {
+        h[u] = vals[e + u * L];
+        c[u] = s.cols[e + u * L];
       }
+      for (int u = 0; u < U; ++u)
+        for (int j = 0; j < PW; ++j) x[u][j] = xs[(int64_t)c[u] * PW + j];
+      for (int u = 0; u < U; ++u)
+        for (int j = 0; j < PW; ++j) acc[j] += h[u] * x[u][j];
     }
-    __syncthreads();
-    for (int64_t t = threadIdx.x; t < n * mm; t += blockDim.x) {
-      const int j = (int)(t / n);
-      const int64_t r = t % n;
-      s.AB[r + (int64_t)(c0 + j0 + j) * n] = s.Y[r * DWG_MB + j];
+#pragma unroll 1
+    for (; e < e1; e += L) {
+      const T hh = vals[e];
+      const T *xx = xs + (int64_t)s.cols[e] * PW;
+      for (int j = 0; j < PW; ++j) acc[j] += hh * xx[j];
     }
-    __syncthreads();
+#pragma unroll 1
+    for (int m = L >> 1; m >= 1; m >>= 1)
+      for (int j = 0; j < PW; ++j) acc[j] += __shfl_xor(acc[j], m);
+    if (r < r1)
+      for (int j = 0; j < PW; ++j)
+        if (j < mm && (j & (L - 1)) == sub) s.AB[r + (int64_t)(c + j) * n] = (double)acc[j];
+  }
+}
+
+template <typename T, int PW>
+__device__ void dwg_apply_chunk(DwgShared &sh, const DwgSolve &s, const DwgParams &prm, const T *vals, int c,
+                                int mm) {
+  T *xs = (T *)(s.X + (int64_t)(sh.phase & 1) * s.n * DWG_MB);
+  dwg_stage<T, PW>(sh, s, xs, c, mm);
+  dwg_sync(sh, s, prm);
+  dwg_hx_rows<T, PW>(sh, s, xs, vals, c, mm);
+  __syncthreads();
+}
+
+// AB[:, c0 .. c0+m) = H B[:, c0 .. c0+m), in chunks of DWG_MB columns
+__device__ void dwg_apply(DwgShared &sh, const DwgSolve &s, const DwgParams &prm, int c0, int m, bool fp64) {
+  for (int j0 = 0; j0 < m; j0 += DWG_MB) {
+    const int mm = min(DWG_MB, m - j0), c = c0 + j0;
+    if (fp64) {
+      if (mm <= 4) dwg_apply_chunk<double, 4>(sh, s, prm, s.v64, c, mm);
+      else if (mm <= 8) dwg_apply_chunk<double, 8>(sh, s, prm, s.v64, c, mm);
+      else dwg_apply_chunk<double, 16>(sh, s, prm, s.v64, c, mm);
+    } else {
+      if (mm <= 4) dwg_apply_chunk<float, 4>(sh, s, prm, s.v32, c, mm);
+      else if (mm <= 8) dwg_apply_chunk<float, 8>(sh, s, prm, s.v32, c, mm);
+      else dwg_apply_chunk<float, 16>(sh, s, prm, s.v32, c, mm);
+    }
   }
   if (threadIdx.x == 0) {
     sh.matvecs += m;
@@ -211,10 +348,10 @@ __device__ void dwg_apply(DwgShared &sh, const DwgSolve &s, int c0, int m, bool 
 }
 
 // S rows/columns of basis vectors [c0, c0+m): S[i][j] = b_i . ab_j, i < c0+m
-__device__ void dwg_update_S(DwgShared &sh, const DwgSolve &s, int c0, int m) {
+__device__ void dwg_update_S(DwgShared &sh, const DwgSolve &s, const DwgParams &prm, int c0, int m) {
   for (int j0 = 0; j0 < m; j0 += DWG_MB) {
     const int mm = min(DWG_MB, m - j0);
-    dwg_gram(sh, s.n, s.B, c0 + j0 + mm, s.AB + (int64_t)(c0 + j0) * s.n, mm);
+    dwg_gram(sh, s, prm, s.B, c0 + j0 + mm, s.AB + (int64_t)(c0 + j0) * s.n, mm);
     for (int o = threadIdx.x; o < (c0 + j0 + mm) * mm; o += blockDim.x) {
       const int i = o / mm, j = c0 + j0 + o % mm;
       if (i <= j) {
@@ -226,7 +363,7 @@ __device__ void dwg_update_S(DwgShared &sh, const DwgSolve &s, int c0, int m) {
   }
 }
 
-__device__ void dwg_eigen(DwgShared &sh, int k) { lds_jacobi<DWG_LIM>(sh, k); }
+__device__ __attribute__((noinline)) void dwg_eigen(DwgShared &sh, int k) { lds_jacobi<DWG_LIM>(sh, k); }
 
 // component r of Ritz vector i (in the current basis)
 __device__ inline double dwg_svec(const DwgShared &sh, int i, int r) { return sh.V[r * DWG_LIM + sh.order[i]]; }
@@ -238,12 +375,10 @@ __device__ void dwg_collapse(DwgShared &sh, const DwgSolve &s, int kp, int q) {
     sh.C1[o] = j < q ? dwg_svec(sh, j, k) : 0.0;
   }
   __syncthreads();
-  dwg_combine(sh, s.n, s.B, nullptr, kp, q, s.R);
-  for (int64_t t = threadIdx.x; t < s.n * q; t += blockDim.x) s.B[t] = s.R[t];
-  __syncthreads();
-  dwg_combine(sh, s.n, s.AB, nullptr, kp, q, s.R);
-  for (int64_t t = threadIdx.x; t < s.n * q; t += blockDim.x) s.AB[t] = s.R[t];
-  __syncthreads();
+  dwg_combine(sh, s, s.B, nullptr, kp, q, s.R);
+  dwg_copy_cols(sh, s, s.B, s.R, q);
+  dwg_combine(sh, s, s.AB, nullptr, kp, q, s.R);
+  dwg_copy_cols(sh, s, s.AB, s.R, q);
 }
 
 // S = diag(eigs[0..q)), V = I: the state right after a collapse
@@ -259,9 +394,10 @@ __device__ void dwg_reset_S(DwgShared &sh, int q) {
 
 // Orthonormalise the q vectors at B[:, p .. p+q) against B[:, 0..p) and among themselves
 // (same passes and thresholds as the host driver's FP64 path); sh.kept, sh.restart
-__device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int q, double ortho, int rounds) {
+__device__ __attribute__((noinline)) void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, const DwgParams &prm, int p, int q,
+                                   int rounds) {
   const int tid = threadIdx.x;
-  const int64_t n = s.n;
+  const int64_t n = s.n, r0 = sh.r0, r1 = sh.r1;
   double *Vp = s.B + (int64_t)p * n;
   if (tid == 0) {
     sh.kept = q;
@@ -276,7 +412,7 @@ __device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int 
       // G = [B V]^T V, chunked over V's columns; W = rows < p, Gv = rows p.. (kept in A)
       for (int j0 = 0; j0 < k; j0 += DWG_MB) {
         const int mm = min(DWG_MB, k - j0);
-        dwg_gram(sh, n, s.B, p + k, Vp + (int64_t)j0 * n, mm);
+        dwg_gram(sh, s, prm, s.B, p + k, Vp + (int64_t)j0 * n, mm);
         for (int o = tid; o < (p + k) * mm; o += blockDim.x) {
           const int a = o / mm, j = j0 + o % mm;
           if (a < p) sh.Wt[a * DWG_LIM + j] = sh.G[o];  // W
@@ -295,7 +431,7 @@ __device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int 
           if (nv > 0)
             for (int a = 0; a < p; ++a) dcur = fmax(dcur, fabs(sh.Wt[a * DWG_LIM + j]) / nv);
         }
-        sh.flag = dcur < ortho ? 1 : (dcur > sh.dprev ? 2 : 0);
+        sh.flag = dcur < prm.ortho ? 1 : (dcur > sh.dprev ? 2 : 0);
         sh.dprev = dcur;
       }
       __syncthreads();
@@ -307,10 +443,20 @@ __device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int 
           sh.C1[o] = j < mm ? sh.Wt[a * DWG_LIM + j0 + j] : 0.0;
         }
         __syncthreads();
-        for (int64_t r = tid; r < n; r += blockDim.x) {
+#pragma unroll 1
+        for (int64_t r = r0 + tid; r < r1; r += blockDim.x) {
           double acc[DWG_MB];
           for (int j = 0; j < DWG_MB; ++j) acc[j] = 0.0;
-          for (int a = 0; a < p; ++a) {
+          int a = 0;
+#pragma unroll 1
+          for (; a + 4 <= p; a += 4) {  // four columns' loads in flight
+            double b[4];
+            for (int u = 0; u < 4; ++u) b[u] = s.B[r + (int64_t)(a + u) * n];
+            for (int u = 0; u < 4; ++u)
+              for (int j = 0; j < DWG_MB; ++j) acc[j] += b[u] * sh.C1[(a + u) * DWG_MB + j];
+          }
+#pragma unroll 1
+          for (; a < p; ++a) {
             const double b = s.B[r + (int64_t)a * n];
             for (int j = 0; j < DWG_MB; ++j) acc[j] += b * sh.C1[a * DWG_MB + j];
           }
@@ -378,9 +524,11 @@ __device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int 
     // V[:, 0..kk) = V[:, keep] T, through R (the kept columns are read before any write)
     for (int j0 = 0; j0 < kk; j0 += DWG_MB) {
       const int mm = min(DWG_MB, kk - j0);
-      for (int64_t r = tid; r < n; r += blockDim.x) {
+#pragma unroll 1
+      for (int64_t r = r0 + tid; r < r1; r += blockDim.x) {
         double acc[DWG_MB];
         for (int j = 0; j < DWG_MB; ++j) acc[j] = 0.0;
+#pragma unroll 1
         for (int a = 0; a < kk; ++a) {
           const double v = Vp[r + (int64_t)sh.keep[a] * n];
           for (int j = 0; j < DWG_MB; ++j)
@@ -391,34 +539,19 @@ __device__ void dwg_orthonormalise(DwgShared &sh, const DwgSolve &s, int p, int 
       }
       __syncthreads();
     }
-    for (int64_t t = tid; t < n * kk; t += blockDim.x) Vp[t] = s.R[t];
-    __syncthreads();
+    dwg_copy_cols(sh, s, Vp, s.R, kk);
   }
 }
 
-// the whole solve of workgroup blockIdx.x
-__global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restrict__ solves, DwgParams prm) {
-  __shared__ DwgShared sh;
-  const DwgSolve s = solves[blockIdx.x];
-  const int tid = threadIdx.x;
-  const int64_t n = s.n;
-  const int nume = s.nume, lim = s.lim;
-  if (tid == 0) {
-    sh.kv = 0;
-    sh.nloops = 0;
-    sh.restarts = 0;
-    sh.ierr = 0;
-    sh.matvecs = 0;
-    sh.matvecs64 = 0;
-    sh.fp64 = prm.mixed ? 0 : 1;
-    sh.first = 1;
-    sh.omin_to = 0;
-  }
-  // ---- start vectors: unit vectors at the nume lowest diagonal elements (ties: lower index)
-  for (int c = 0; c < nume; ++c) {
+// Start vectors: the nume lowest diagonal elements (ties: lower index), chosen by every
+// rank from the team's per-rank lists; sh.chosen.
+__device__ void dwg_start_rows(DwgShared &sh, const DwgSolve &s, const DwgParams &prm) {
+  const int tid = threadIdx.x, nume = s.nume;
+  const int64_t r0 = sh.r0, r1 = sh.r1;
+  for (int c = 0; c < nume; ++c) {  // own rows: repeated selection
     double best = 1e308;
     int bi = 0x7fffffff;
-    for (int64_t r = tid; r < n; r += blockDim.x) {
+    for (int64_t r = r0 + tid; r < r1; r += blockDim.x) {
       bool taken = false;
       for (int l = 0; l < c; ++l) taken |= (sh.chosen[l] == (int)r);
       const double v = s.diag[r];
@@ -437,15 +570,93 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
           bi = sh.mini[t];
         }
       sh.chosen[c] = bi;
+      sh.base[c] = best;
     }
     __syncthreads();
   }
-  for (int64_t t = tid; t < n * nume; t += blockDim.x) s.B[t] = (t % n == sh.chosen[t / n]) ? 1.0 : 0.0;
+  if (s.G == 1) return;
+  const int par = sh.phase & 1;
+  double *mine = s.part + ((int64_t)par * s.G + sh.rank) * DWG_PART;
+  for (int c = tid; c < nume; c += blockDim.x) {
+    mine[2 * c] = sh.base[c];
+    mine[2 * c + 1] = (double)sh.chosen[c];
+  }
+  dwg_sync(sh, s, prm);
+  if (tid == 0) {
+    const double *all = s.part + (int64_t)par * s.G * DWG_PART;
+    for (int c = 0; c < nume; ++c) {
+      double best = 1e308;
+      int bi = 0x7fffffff;
+      for (int g = 0; g < s.G; ++g)
+        for (int l = 0; l < nume; ++l) {
+          const double v = all[(int64_t)g * DWG_PART + 2 * l];
+          const int i = (int)all[(int64_t)g * DWG_PART + 2 * l + 1];
+          if (i == 0x7fffffff) continue;
+          bool taken = false;
+          for (int u = 0; u < c; ++u) taken |= (sh.add[u] == i);
+          if (!taken && (v < best || (v == best && i < bi))) {
+            best = v;
+            bi = i;
+          }
+        }
+      sh.add[c] = bi;
+    }
+    for (int c = 0; c < nume; ++c) sh.chosen[c] = sh.add[c];
+  }
   __syncthreads();
-  dwg_apply(sh, s, 0, nume, sh.fp64);
+}
+
+// switch from the FP32 product to FP64 refinement: re-orthonormalise the basis, H.B and S
+// in FP64; the caller breaks out when the iteration cap is reached
+__device__ void dwg_to_fp64(DwgShared &sh, const DwgSolve &s, const DwgParams &prm) {
+  const int tid = threadIdx.x;
+  dwg_orthonormalise(sh, s, prm, 0, sh.kpass, 2);
+  if (tid == 0) sh.kpass = sh.kept;
+  __syncthreads();
+  dwg_apply(sh, s, prm, 0, sh.kpass, true);
   for (int o = tid; o < DWG_LIM * DWG_LIM; o += blockDim.x) sh.S[o] = 0.0;
   __syncthreads();
-  dwg_update_S(sh, s, 0, nume);
+  dwg_update_S(sh, s, prm, 0, sh.kpass);
+  if (tid == 0) {
+    sh.nncv = sh.kpass;
+    for (int i = 0; i < s.nume; ++i) {
+      sh.icv[i] = 0;
+      sh.eigs[i] = 1e30;
+    }
+    sh.kv = 0;
+    sh.fp64 = 1;
+    sh.omin_to = 0;
+    sh.first = 1;
+    ++sh.nloops;
+  }
+  __syncthreads();
+}
+
+// the whole solve, as rank sh.rank of its team (sh.r0, sh.r1, sh.phase, sh.aborted set)
+__device__ void dwg_solve(DwgShared &sh, const DwgSolve &s, const DwgParams &prm) {
+  const int tid = threadIdx.x;
+  const int64_t n = s.n, r0 = sh.r0, r1 = sh.r1;
+  const int nume = s.nume, lim = s.lim;
+  if (tid == 0) {
+    sh.kv = 0;
+    sh.nloops = 0;
+    sh.restarts = 0;
+    sh.ierr = 0;
+    sh.matvecs = 0;
+    sh.matvecs64 = 0;
+    sh.fp64 = prm.mixed ? 0 : 1;
+    sh.first = 1;
+    sh.omin_to = 0;
+  }
+  __syncthreads();
+  dwg_start_rows(sh, s, prm);
+  for (int c = 0; c < nume; ++c)
+    for (int64_t r = r0 + tid; r < r1; r += blockDim.x) s.B[r + (int64_t)c * n] = (r == sh.chosen[c]) ? 1.0 : 0.0;
+  __syncthreads();
+  dwg_apply(sh, s, prm, 0, nume, sh.fp64);
+  for (int o = tid; o < DWG_LIM * DWG_LIM; o += blockDim.x) sh.S[o] = 0.0;
+  __syncthreads();
+  dwg_update_S(sh, s, prm, 0, nume);
   if (tid == 0) {
     sh.kpass = nume;
     sh.nncv = nume;
@@ -457,8 +668,8 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
     }
   }
   __syncthreads();
-  for (int verify = 0;; ++verify) {
-    while (true) {
+  for (int verify = 0; !sh.aborted; ++verify) {
+    while (!sh.aborted) {
       // ---- Rayleigh-Ritz and Test_Convergence
       if (tid == 0)
         for (int i = 0; i < nume; ++i) sh.old[i] = sh.eigs[i];
@@ -509,27 +720,7 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
         if (tid == 0) sh.omin_to = 1;
         __syncthreads();
         if (!sh.fp64) {
-          // switch to FP64 refinement: re-orthonormalise the basis, H.B and S in FP64
-          dwg_orthonormalise(sh, s, 0, sh.kpass, prm.ortho, 2);
-          if (tid == 0) sh.kpass = sh.kept;
-          __syncthreads();
-          dwg_apply(sh, s, 0, sh.kpass, true);
-          for (int o = tid; o < DWG_LIM * DWG_LIM; o += blockDim.x) sh.S[o] = 0.0;
-          __syncthreads();
-          dwg_update_S(sh, s, 0, sh.kpass);
-          if (tid == 0) {
-            sh.nncv = sh.kpass;
-            for (int i = 0; i < nume; ++i) {
-              sh.icv[i] = 0;
-              sh.eigs[i] = 1e30;
-            }
-            sh.kv = 0;
-            sh.fp64 = 1;
-            sh.omin_to = 0;
-            sh.first = 1;
-            ++sh.nloops;
-          }
-          __syncthreads();
+          dwg_to_fp64(sh, s, prm);
           if (sh.nloops >= s.maxiter) break;
           continue;
         }
@@ -563,8 +754,8 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
         sh.C2[o] = -l * v;
       }
       __syncthreads();
-      dwg_combine(sh, n, s.AB, s.B, kpass, ncand, s.R);
-      dwg_gram(sh, n, s.R, ncand, s.R, ncand);
+      dwg_combine(sh, s, s.AB, s.B, kpass, ncand, s.R);
+      dwg_gram(sh, s, prm, s.R, ncand, s.R, ncand);
       if (tid == 0) {
         const int limadd = min(lim, s.mblock + kpass);
         int nadded = 0, icvc = 0, all_conv = 0;
@@ -594,26 +785,7 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
         if (tid == 0) sh.omin_to = 1;
         __syncthreads();
         if (!sh.fp64) {
-          dwg_orthonormalise(sh, s, 0, sh.kpass, prm.ortho, 2);
-          if (tid == 0) sh.kpass = sh.kept;
-          __syncthreads();
-          dwg_apply(sh, s, 0, sh.kpass, true);
-          for (int o = tid; o < DWG_LIM * DWG_LIM; o += blockDim.x) sh.S[o] = 0.0;
-          __syncthreads();
-          dwg_update_S(sh, s, 0, sh.kpass);
-          if (tid == 0) {
-            sh.nncv = sh.kpass;
-            for (int i = 0; i < nume; ++i) {
-              sh.icv[i] = 0;
-              sh.eigs[i] = 1e30;
-            }
-            sh.kv = 0;
-            sh.fp64 = 1;
-            sh.omin_to = 0;
-            sh.first = 1;
-            ++sh.nloops;
-          }
-          __syncthreads();
+          dwg_to_fp64(sh, s, prm);
           if (sh.nloops >= s.maxiter) break;
           continue;
         }
@@ -621,15 +793,18 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
       }
       // ---- new vectors: preconditioned residuals into B[:, kpass ..)
       const int nadded = sh.nadded;
-      for (int64_t t = tid; t < n * nadded; t += blockDim.x) {
-        const int a = (int)(t / n);
-        const int64_t r = t % n;
-        const double dg = sh.lam[a] - s.diag[r];
-        const double v = s.R[r + (int64_t)sh.add[a] * n];
-        s.B[r + (int64_t)(kpass + a) * n] = fabs(dg) > 1.0e-13 ? v / dg : v / 1.0e-13;
+      for (int a = 0; a < nadded; ++a) {
+        const double lam = sh.lam[a];
+        const double *Ra = s.R + (int64_t)sh.add[a] * n;
+        double *Ba = s.B + (int64_t)(kpass + a) * n;
+        for (int64_t r = r0 + tid; r < r1; r += blockDim.x) {
+          const double dg = lam - s.diag[r];
+          const double v = Ra[r];
+          Ba[r] = fabs(dg) > 1.0e-13 ? v / dg : v / 1.0e-13;
+        }
       }
       __syncthreads();
-      dwg_orthonormalise(sh, s, kpass, nadded, prm.ortho, sh.fp64 ? 2 : 1);
+      dwg_orthonormalise(sh, s, prm, kpass, nadded, sh.fp64 ? 2 : 1);
       if (tid == 0) {
         sh.nncv = sh.kept;
         sh.done = 0;
@@ -662,34 +837,15 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
       }
       if (sh.done == 3) {
         if (!sh.fp64) {
-          dwg_orthonormalise(sh, s, 0, sh.kpass, prm.ortho, 2);
-          if (tid == 0) sh.kpass = sh.kept;
-          __syncthreads();
-          dwg_apply(sh, s, 0, sh.kpass, true);
-          for (int o = tid; o < DWG_LIM * DWG_LIM; o += blockDim.x) sh.S[o] = 0.0;
-          __syncthreads();
-          dwg_update_S(sh, s, 0, sh.kpass);
-          if (tid == 0) {
-            sh.nncv = sh.kpass;
-            for (int i = 0; i < nume; ++i) {
-              sh.icv[i] = 0;
-              sh.eigs[i] = 1e30;
-            }
-            sh.kv = 0;
-            sh.fp64 = 1;
-            sh.omin_to = 0;
-            sh.first = 1;
-            ++sh.nloops;
-          }
-          __syncthreads();
+          dwg_to_fp64(sh, s, prm);
           if (sh.nloops >= s.maxiter) break;
           continue;
         }
         break;
       }
       // ---- Add_Vectors_To_AB_S
-      dwg_apply(sh, s, kpass, sh.nncv, sh.fp64);
-      dwg_update_S(sh, s, kpass, sh.nncv);
+      dwg_apply(sh, s, prm, kpass, sh.nncv, sh.fp64);
+      dwg_update_S(sh, s, prm, kpass, sh.nncv);
       if (tid == 0) {
         sh.kpass = kpass + sh.nncv;
         ++sh.nloops;
@@ -711,8 +867,8 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
       sh.C2[o] = (k == j) ? -sh.eigs[j] : 0.0;
     }
     __syncthreads();
-    dwg_combine(sh, n, s.AB, s.B, nume, nume, s.R);
-    dwg_gram(sh, n, s.R, nume, s.R, nume);
+    dwg_combine(sh, s, s.AB, s.B, nume, nume, s.R);
+    dwg_gram(sh, s, prm, s.R, nume, s.R, nume);
     if (tid == 0) {
       for (int i = 0; i < nume; ++i) sh.rnorm[i] = sqrt(fmax(sh.G[i * nume + i], 0.0));
       int reopen = 0;
@@ -736,7 +892,8 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (tid == 0 && sh.rank == 0) {
+    if (sh.aborted) sh.ierr |= DWG_ABORTED;
     for (int i = 0; i < nume; ++i) {
       s.evals[i] = sh.eigs[i];
       s.dE[i] = sh.old[i];
@@ -748,12 +905,51 @@ __global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restri
     s.info[3] = sh.restarts;
     s.info[4] = sh.ierr;
     // converged: stopped before maxiter and, under the strict test, every returned
-    // residual below critr (an orthogonalisation failure or the reopen cap counts as not
-    // converged)
+    // residual below critr (an orthogonalisation failure, the reopen cap or a timed-out
+    // team barrier counts as not converged)
     int conv = (sh.nloops >= s.maxiter && !sh.omin_to) ? 0 : 1;
     if (prm.strict)
       for (int i = 0; i < nume; ++i)
         if (!(sh.rnorm[i] <= s.critr)) conv = 0;
+    if (sh.aborted) conv = 0;
     s.info[5] = conv;
   }
+}
+
+// Persistent loop of one workgroup: take the next seat (one atomic per workgroup), run that
+// rank of its solve, until the seats run out.  solves[] ascends in seat0.
+__device__ void dwg_run(DwgShared &sh, const DwgSolve *__restrict__ solves, int nsolves, int nseats,
+                        unsigned *claim, const DwgParams &prm) {
+  if (threadIdx.x == 0) sh.prm = prm;
+  for (;;) {
+    if (threadIdx.x == 0) sh.seat = (int)atomicAdd(claim, 1u);
+    __syncthreads();
+    const int seat = sh.seat;
+    if (seat >= nseats) return;
+    int lo = 0, hi = nsolves - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (solves[mid].seat0 <= seat) lo = mid;
+      else hi = mid - 1;
+    }
+    if (threadIdx.x == 0) {
+      sh.sol = solves[lo];
+      const DwgSolve &s = sh.sol;
+      const int64_t chunk = ((s.n + s.G - 1) / s.G + 63) / 64 * 64;
+      sh.rank = seat - s.seat0;
+      sh.r0 = min(s.n, (int64_t)sh.rank * chunk);
+      sh.r1 = min(s.n, sh.r0 + chunk);
+      sh.phase = 0;
+      sh.aborted = 0;
+    }
+    __syncthreads();
+    dwg_solve(sh, sh.sol, sh.prm);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(DWG_THREADS) void k_dav_wg(const DwgSolve *__restrict__ solves, int nsolves, int nseats,
+                                                         unsigned *claim, DwgParams prm) {
+  __shared__ DwgShared sh;
+  dwg_run(sh, solves, nsolves, nseats, claim, prm);
 }

@@ -59,13 +59,13 @@ __device__ void lds_jacobi(SH &sh, int k) {
   }
   __syncthreads();
   const double jtol = sh.dctl;
-  const int K = k + (k & 1);
-  double *Ac = A, *Vc = V, *An = A2, *Vn = V2;  // current and next (swapped each round)
+  const int K = k + (k & 1), H = K / 2;
   for (int sweep = 0; sweep < 40 && k > 1; ++sweep) {
     if (tid == 0) sh.rot = 0;
     __syncthreads();
     for (int rr = 0; rr < K - 1; ++rr) {
-      for (int i = tid; i < K / 2; i += blockDim.x) {
+      // the round's pairs and their rotations
+      for (int i = tid; i < H; i += blockDim.x) {
         int p, q;
         if (i == 0) {
           p = K - 1;
@@ -81,7 +81,7 @@ __device__ void lds_jacobi(SH &sh, int k) {
         }
         double c = 1.0, sn = 0.0;
         if (q < k) {
-          const double apq = Ac[p * LIM + q], app = Ac[p * LIM + p], aqq = Ac[q * LIM + q];
+          const double apq = A[p * LIM + q], app = A[p * LIM + p], aqq = A[q * LIM + q];
           if (fabs(apq) > jtol) {
             // rotation zeroing A[p][q]: tan = t solves t^2 + 2 t theta - 1 = 0, smaller root
             const double theta = (aqq - app) / (2.0 * apq);
@@ -90,54 +90,49 @@ __device__ void lds_jacobi(SH &sh, int k) {
             sn = t * c;
             sh.rot = 1;
           }
-          sh.partner[p] = q;
-          sh.partner[q] = p;
-          sh.pairidx[p] = i;
-          sh.pairidx[q] = i;
-        } else {
-          if (p < k) {
-            sh.partner[p] = -1;
-            sh.pairidx[p] = i;
-          }
         }
+        sh.partner[i] = p;  // pair i: players p < q (q >= k: p sits the round out)
+        sh.pairidx[i] = q;
         sh.jc[i] = c;
         sh.js[i] = sn;
       }
       __syncthreads();
-      // A' = J^T A J, V' = V J with J[p][p] = J[q][q] = c, J[p][q] = s, J[q][p] = -s (p < q)
-      for (int o = tid; o < k * k; o += blockDim.x) {
-        const int x = o / k, y = o % k;
-        const int px = sh.partner[x], py = sh.partner[y];
-        const double cx = sh.jc[sh.pairidx[x]], sx = sh.js[sh.pairidx[x]];
-        const double cy = sh.jc[sh.pairidx[y]], sy = sh.js[sh.pairidx[y]];
-        // column y of A J: (A J)[r][y] = c_y A[r][y] + (y is q ? s : -s) A[r][py]
-        const double fy = (py < 0) ? 0.0 : (py < y ? sy : -sy);
-        const double fx = (px < 0) ? 0.0 : (px < x ? sx : -sx);
-        const double ay = cy * Ac[x * LIM + y] + (py < 0 ? 0.0 : fy * Ac[x * LIM + py]);
-        double v = cx * ay;
-        if (px >= 0) v += fx * (cy * Ac[px * LIM + y] + (py < 0 ? 0.0 : fy * Ac[px * LIM + py]));
-        An[x * LIM + y] = v;
-        Vn[x * LIM + y] = cy * Vc[x * LIM + y] + (py < 0 ? 0.0 : fy * Vc[x * LIM + py]);
+      // A = J^T A J and V = V J with J[p][p] = J[q][q] = c, J[p][q] = s, J[q][p] = -s, in
+      // place: the pairs split A into disjoint 2 x 2 blocks (rows of pair i, columns of pair
+      // j) and V into disjoint row pieces; a block whose pairs both keep c = 1, s = 0 stays
+      for (int o = tid; o < H * H + k * H; o += blockDim.x) {
+        if (o < H * H) {
+          const int i = o / H, j = o % H;
+          const double si = sh.js[i], sj = sh.js[j];
+          if (si == 0.0 && sj == 0.0) continue;
+          const double ci = sh.jc[i], cj = sh.jc[j];
+          const int pi = sh.partner[i], qi = sh.pairidx[i], pj = sh.partner[j], qj = sh.pairidx[j];
+          const bool hi = qi < k, hj = qj < k;
+          const double a_pp = A[pi * LIM + pj], a_pq = hj ? A[pi * LIM + qj] : 0.0;
+          const double a_qp = hi ? A[qi * LIM + pj] : 0.0, a_qq = (hi && hj) ? A[qi * LIM + qj] : 0.0;
+          // right by J_j, then left by J_i^T
+          const double b_pp = cj * a_pp - sj * a_pq, b_pq = sj * a_pp + cj * a_pq;
+          const double b_qp = cj * a_qp - sj * a_qq, b_qq = sj * a_qp + cj * a_qq;
+          A[pi * LIM + pj] = ci * b_pp - si * b_qp;
+          if (hj) A[pi * LIM + qj] = ci * b_pq - si * b_qq;
+          if (hi) A[qi * LIM + pj] = si * b_pp + ci * b_qp;
+          if (hi && hj) A[qi * LIM + qj] = si * b_pq + ci * b_qq;
+        } else {
+          const int x = (o - H * H) / H, j = (o - H * H) % H;
+          const double sj = sh.js[j];
+          if (sj == 0.0) continue;
+          const double cj = sh.jc[j];
+          const int pj = sh.partner[j], qj = sh.pairidx[j];  // sj != 0: both players present
+          const double vp = V[x * LIM + pj], vq = V[x * LIM + qj];
+          V[x * LIM + pj] = cj * vp - sj * vq;
+          V[x * LIM + qj] = sj * vp + cj * vq;
+        }
       }
       __syncthreads();
-      double *t = Ac;
-      Ac = An;
-      An = t;
-      t = Vc;
-      Vc = Vn;
-      Vn = t;
     }
     const int rot = sh.rot;
     __syncthreads();
     if (!rot) break;
-  }
-  if (Ac != A) {  // an odd number of rounds left the result in the second buffers
-    for (int o = tid; o < k * k; o += blockDim.x) {
-      const int x = o / k, y = o % k;
-      A[x * LIM + y] = Ac[x * LIM + y];
-      V[x * LIM + y] = Vc[x * LIM + y];
-    }
-    __syncthreads();
   }
   // ascending order, lower column first on ties: column e goes to its rank
   for (int e = tid; e < k; e += blockDim.x) {

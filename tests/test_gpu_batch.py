@@ -115,15 +115,18 @@ def test_single_launch_matches_dense(gpu, precision):
 
 
 def test_single_launch_same_alone_and_in_mix(gpu):
-    """A workgroup's solve does not depend on the other solves of the launch."""
+    """A solve of the persistent launch does not depend on the other solves of the launch,
+    whether one workgroup runs it or a team (n > 2048: two workgroups or more)."""
     xmls = random_sd_systems(16, 47)
-    systems = [s for s in (gpu.System.from_xml(x) for x in xmls) if 0 < s.dim <= 4000]
+    systems = [s for s in (gpu.System.from_xml(x) for x in xmls) if 0 < s.dim <= 6000]
+    assert any(s.dim > 2048 for s in systems) and any(s.dim <= 2048 for s in systems)
     nume = [min(4, s.dim) for s in systems]
     b = gpu.Batch(systems)
     b.set_single_launch(1 << 15)
     ev, rs, _ = b.solve(nume, critr=1e-6, precision=gpu.PREC_FP32)
     b.close()
-    for i in (0, len(systems) // 2, len(systems) - 1):
+    picks = {0, len(systems) // 2, len(systems) - 1, int(np.argmax([s.dim for s in systems]))}
+    for i in sorted(picks):
         one = gpu.Batch([systems[i]])
         one.set_single_launch(1 << 15)
         e1, r1, _ = one.solve([nume[i]], critr=1e-6, precision=gpu.PREC_FP32)
@@ -183,12 +186,15 @@ def test_server_runs_unstorable_h_on_the_product_form(gpu, tmp_path):
         gpu.footprint(one, gpu.OP_FACTORED)
 
 
-def test_batch_256_uneven_sd_solves_vs_oracle(gpu):
+@pytest.mark.parametrize("one_launch", [False, True])
+def test_batch_256_uneven_sd_solves_vs_oracle(gpu, one_launch):
     """The batched workload at its stated size: 256 sd-shell solves, Nv and Zv uniform in
     1..6, 2Jz from the allowed range, each with its own seeded interaction (n 18..93,710),
-    5 pairs each, FP32 H.V with FP64 refinement to residual 1e-6, in one batch call.
-    Every eigenvalue against the oracle's (dense / Lanczos on the oracle's H,
-    tests/golden/batch256_oracle.json) within 1e-6 MeV, every residual <= 1e-6."""
+    5 pairs each, FP32 H.V with FP64 refinement to residual 1e-6, in one batch call: with
+    the default split (solves up to n = 16384 in one persistent launch, the rest
+    host-driven beside it) and with all 256 in the one launch (teams of up to 46
+    workgroups).  Every eigenvalue against the oracle's (dense / Lanczos on the oracle's
+    H, tests/golden/batch256_oracle.json) within 1e-6 MeV, every residual <= 1e-6."""
     import json
     import os
     from conftest import GOLDEN
@@ -198,6 +204,8 @@ def test_batch_256_uneven_sd_solves_vs_oracle(gpu):
     assert [s.dim for s in systems] == [x["n"] for x in ref]
     nume = [min(5, s.dim) for s in systems]
     b = gpu.Batch(systems)
+    if one_launch:
+        b.set_single_launch(1 << 30)
     ev, rs, st = b.solve(nume, critr=1e-6, precision=gpu.PREC_FP32)
     b.close()
     assert st["converged"]
