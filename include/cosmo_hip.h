@@ -257,18 +257,18 @@ int cosmo_read_ee(const char *path, int64_t n, int maxcount, int *count, double 
 /* ---- batched solves ------------------------------------------------------------------ */
 /* Independent systems solved together (DavidsonDiagonalization per system,
  * include/kdav/davidson.cxx:167-223, with a wider basis: lim = the larger of p->lim,
- * nume + max(20, nume) and min(nume + 40, 45), at most n; at least 4000 iterations).  Solves of dimension <= the single-launch bound (default 2048,
- * at most 32768) with lim <= 48 and nume, block <= 16 on stored operators run to
- * completion in ONE kernel launch, one workgroup per solve; the others run concurrently
- * from worker threads, each on its own stream with the host-driven Davidson.  One
- * workgroup streams a few hundred GB/s at most, so the single-launch path pays only where
- * the host-driven loop's per-iteration cost dominates (small n).  Results land per solve
- * (evals and resid: nume_per_solve[i] entries each, concatenated). */
+ * nume + max(20, nume) and min(nume + 40, 45), at most n; at least 4000 iterations).  Solves of dimension <= the single-launch bound (default
+ * 16384, at most 2^30) with lim <= 48 and nume, block <= 16 on stored operators run to
+ * completion in ONE persistent kernel launch, each by a team of workgroups (one per 2048
+ * rows, at most 128); the others run concurrently from worker threads, each on its own
+ * stream with the host-driven Davidson.  Results land per solve (evals and resid:
+ * nume_per_solve[i] entries each, concatenated). */
 int cosmo_batch_create(int nsys, const cosmo_system *systems, int flags, cosmo_batch *out);
 int cosmo_batch_solve(cosmo_batch b, const cosmo_dav_params *p, const int *nume_per_solve, double *evals,
                       double *resid, cosmo_dav_stats *stats);
-/* Dimension bound of the single-launch path (0 turns it off).  No reference counterpart:
- * a batching knob of this build. */
+/* Dimension bound of the single-launch path (0 turns it off; 1 << 30 puts every solve
+ * within the kernel's bounds in the one launch).  No reference counterpart: a batching
+ * knob of this build. */
 int cosmo_batch_set_single_launch(cosmo_batch b, int64_t nmax);
 int cosmo_batch_destroy(cosmo_batch b);
 
