@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--profile-solve", choices=["fp64", "mixed"], default=None,
                     help="only one 10-pair product-form solve (block 16, basis limit 100) in this precision, "
                          "after one warm-up solve; for kernel profiles")
+    ap.add_argument("--profile-batch", choices=["split", "one"], default=None,
+                    help="only the 256-solve batch, after one warm-up call: default split, or every solve "
+                         "in the one persistent launch; for kernel profiles")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 4 GiB device copy three times (counter calibration)")
     args = ap.parse_args()
@@ -112,6 +115,13 @@ def main():
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": x.numel() * 4, "launches": 3}))
         return
 
+    if args.profile_batch:
+        one = args.profile_batch == "one"
+        batch_section(ca, synth, one_launch=one, warm=True, also_one=False)
+        out = batch_section(ca, synth, one_launch=one, warm=False, also_one=False)
+        print(json.dumps({"batch": args.profile_batch, "seconds": out["solve_s"], "solves": out["solves"],
+                          "max_resid": out["max_resid"], "converged": out["converged"]}))
+        return
     tmp = tempfile.mkdtemp(prefix="cosmo_bench_")
     name = "%s%dp%dn" % (args.shell, args.zv, args.nv)
     xml = synth.write_system(tmp, args.shell, name, args.nv, args.zv, 0, "+", args.seed)
@@ -225,6 +235,7 @@ def main():
     del X, Y
     fop = None  # frees the product-form operator
     if not args.no_batch:
+        batch_section(ca, synth, warm=True, also_one=False)  # warm-up: kernel loading, pools
         out["batch"] = batch_section(ca, synth)
     if not args.no_serve:
         out["serve"] = serve_section(ca, synth)
@@ -474,8 +485,10 @@ def cpu_reference_solve(d, name, bloc):
             "resid": [float(x) for x in R[:bloc]]}
 
 
-def batch_section(ca, synth):
-    """256 independent sd-shell solves of uneven size in one batch, mixed precision."""
+def batch_section(ca, synth, one_launch=False, warm=False, also_one=True):
+    """256 independent sd-shell solves of uneven size in one batch, mixed precision: the
+    default split (solves up to n = 16384 in one persistent launch, the rest host-driven
+    beside it), then, unless also_one is False, all 256 in the one launch."""
     import numpy as np
     t0 = time.perf_counter()
     reqs, _ = synth.random_requests(ca, 256, 17)
@@ -486,6 +499,8 @@ def batch_section(ca, synth):
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = ca.Batch(systems)
+    if one_launch:
+        b.set_single_launch(1 << 30)
     t_build = time.perf_counter() - t0
     print("[bench] batch: operators built in %.2fs, solving" % t_build, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -494,13 +509,25 @@ def batch_section(ca, synth):
     b.close()
     dims = np.array([s.dim for s in systems])
     maxres = float(max(np.max(r) for r in rs))
-    print("[bench] batch: 256 solves (n %d..%d) built %.2fs solved %.2fs" % (dims.min(), dims.max(), t_build, dt),
-          file=sys.stderr, flush=True)
+    print("[bench] batch: 256 solves (n %d..%d) built %.2fs solved %.2fs%s" %
+          (dims.min(), dims.max(), t_build, dt, " (warm-up)" if warm else ""), file=sys.stderr, flush=True)
+    one = None
+    if also_one:
+        b1 = ca.Batch(systems)
+        b1.set_single_launch(1 << 30)
+        t0 = time.perf_counter()
+        _, rs1, st1 = b1.solve(nume, critr=1e-6, precision=ca.PREC_FP32)
+        one = {"solve_s": round(time.perf_counter() - t0, 3), "converged": st1["converged"],
+               "max_resid": float(max(np.max(r) for r in rs1))}
+        b1.close()
+        print("[bench] batch: all 256 in one launch %.2fs" % one["solve_s"], file=sys.stderr, flush=True)
     return {"solves": len(systems), "dims_min": int(dims.min()), "dims_median": int(np.median(dims)),
             "dims_max": int(dims.max()), "eigenpairs": int(sum(nume)), "setup_s": round(t_setup, 3),
             "build_s": round(t_build, 3), "solve_s": round(dt, 3), "solves_per_s": round(len(systems) / dt, 2),
             "eigenpairs_per_s": round(sum(nume) / dt, 2), "max_resid": maxres, "converged": st["converged"],
-            "precision": "mixed (FP32 H.V, FP64 refinement)"}
+            "precision": "mixed (FP32 H.V, FP64 refinement)",
+            "one_launch": "all solves up to n = 16384 in one persistent launch, the rest host-driven beside it",
+            "all_in_one_launch": one}
 
 
 def serve_section(ca, synth, count=48, rate=12.0):
