@@ -606,10 +606,16 @@ __device__ void dwg_start_rows(DwgShared &sh, const DwgSolve &s, const DwgParams
   __syncthreads();
 }
 
-// switch from the FP32 product to FP64 refinement: re-orthonormalise the basis, H.B and S
-// in FP64; the caller breaks out when the iteration cap is reached
+// switch from the FP32 product to FP64 refinement: collapse the basis to the wanted Ritz
+// vectors (current in sh.V at every switch point), re-orthonormalise them, H.B and S in
+// FP64; the caller breaks out when the iteration cap is reached
 __device__ void dwg_to_fp64(DwgShared &sh, const DwgSolve &s, const DwgParams &prm) {
   const int tid = threadIdx.x;
+  if (sh.kpass > s.nume) {
+    dwg_collapse(sh, s, sh.kpass, s.nume);
+    if (tid == 0) sh.kpass = s.nume;
+    __syncthreads();
+  }
   dwg_orthonormalise(sh, s, prm, 0, sh.kpass, 2);
   if (tid == 0) sh.kpass = sh.kept;
   __syncthreads();

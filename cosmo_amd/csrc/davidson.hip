@@ -17,9 +17,10 @@
 //
 // Mixed precision (COSMO_PREC_FP32): H.X runs on the FP32 values with FP32 vectors, and
 // the basis, its image and the work blocks are stored in FP32 (all sums in FP64), until
-// every wanted residual drops under mixed_switch * max(1, |lambda|); the whole basis is
-// then widened to FP64 and re-orthonormalised, H.X is recomputed in FP64 and the
-// iteration continues in FP64 to the reference thresholds (the FP64 refinement).
+// every wanted residual drops under mixed_switch * max(1, |lambda|); the basis is then
+// collapsed to the wanted Ritz vectors, widened to FP64 and re-orthonormalised, H.X is
+// recomputed in FP64 and the iteration continues in FP64 to the reference thresholds (the
+// FP64 refinement).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -957,9 +958,15 @@ int cosmo_davidson(cosmo_op op, const cosmo_dav_params *pp, const double *init, 
   int nloops = 0, ierr = 0, restarts = 0;
   // mixed precision: widen the basis to FP64, recompute H.B in FP64, continue in FP64
   auto switch_fp64 = [&]() -> int {
-    // keep the whole FP32-grown basis; it is re-orthonormalised in FP64 (vectors added
-    // with a single round may have drifted, which matters once the basis nears the whole
-    // space), then H.B and S are recomputed in FP64
+    // collapse the FP32-grown basis to the nume wanted Ritz vectors (H.X is recomputed in
+    // FP64 for these only: on the fp 4p4n 10-pair solve 319 FP64 products instead of 372,
+    // 0.98 s instead of 1.02 s; keeping the whole basis or 32 / 48 vectors was slower), re-
+    // orthonormalise them in FP64, then recompute H.B and S in FP64
+    if (kpass > nume) {
+      int r0 = collapse(D, svec, kpass);
+      if (r0) return r0;
+      kpass = nume;
+    }
     int r = convert_cols(D, D.B32, 4, D.B, kpass);
     if (r) return r;
     D.use64();
