@@ -4,7 +4,8 @@
 // dav_wg.h bounds) run to completion in ONE persistent kernel launch once worker threads have
 // built their operators, each solve by a team of workgroups sized to its dimension (one
 // workgroup per 2048 rows); the others own their operator and Davidson blocks and run on
-// their own HIP streams from the worker threads, beside the launch.  All kernels are
+// their own HIP streams from the worker threads at the same time (their kernels take CUs as
+// the launch's workgroups retire: each of those fills its CU's registers).  All kernels are
 // deterministic (no atomics in H.X, reductions in fixed order, team sizes fixed by the
 // system), so a solve gives the same bits alone and in a mix on either path.  The server
 // admits queued requests in arrival order as
