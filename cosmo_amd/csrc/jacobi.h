@@ -61,7 +61,15 @@ __device__ void lds_jacobi(SH &sh, int k) {
   const double jtol = sh.dctl;
   const int K = k + (k & 1), H = K / 2;
   for (int sweep = 0; sweep < 40 && k > 1; ++sweep) {
+    // stop before a sweep that would rotate nothing: one pass over the off-diagonal part
     if (tid == 0) sh.rot = 0;
+    __syncthreads();
+    for (int o = tid; o < k * k; o += blockDim.x) {
+      const int r = o / k, c = o % k;
+      if (r < c && fabs(A[r * LIM + c]) > jtol) sh.rot = 1;
+    }
+    __syncthreads();
+    if (!sh.rot) break;
     __syncthreads();
     for (int rr = 0; rr < K - 1; ++rr) {
       // the round's pairs and their rotations
@@ -88,7 +96,6 @@ __device__ void lds_jacobi(SH &sh, int k) {
             const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
             sn = t * c;
-            sh.rot = 1;
           }
         }
         sh.partner[i] = p;  // pair i: players p < q (q >= k: p sits the round out)
@@ -130,9 +137,6 @@ __device__ void lds_jacobi(SH &sh, int k) {
       }
       __syncthreads();
     }
-    const int rot = sh.rot;
-    __syncthreads();
-    if (!rot) break;
   }
   // ascending order, lower column first on ties: column e goes to its rank
   for (int e = tid; e < k; e += blockDim.x) {
