@@ -262,7 +262,9 @@ int cosmo_read_ee(const char *path, int64_t n, int maxcount, int *count, double 
  * completion in ONE persistent kernel launch, each by a team of workgroups (one per 2048
  * rows, at most 128); the others run concurrently from worker threads, each on its own
  * stream with the host-driven Davidson.  Results land per solve (evals and resid:
- * nume_per_solve[i] entries each, concatenated). */
+ * nume_per_solve[i] entries each, concatenated).  COSMO_ECONV: a solve stopped above its
+ * residual target; COSMO_EHIP: a team of the launch waited more than 10 s at a barrier
+ * with all its workgroups seated (the launch then ends without waiting further). */
 int cosmo_batch_create(int nsys, const cosmo_system *systems, int flags, cosmo_batch *out);
 int cosmo_batch_solve(cosmo_batch b, const cosmo_dav_params *p, const int *nume_per_solve, double *evals,
                       double *resid, cosmo_dav_stats *stats);
