@@ -226,7 +226,10 @@ def test_serving_poisson_sd_fp_mixed_stream(gpu):
     requests, Nv, Zv and 2Jz drawn as in the batch, k uniform in 1..10 pairs, residual
     tolerance log-uniform in [1e-6, 1e-4], served in mixed precision while others run.
     Each result equals the same request solved alone (same precision and parameters) and
-    lies within 1 keV of an all-FP64 solve alone; every residual meets its tolerance."""
+    lies within 1 keV of an all-FP64 solve alone; every residual meets its tolerance; and
+    for the requests of dimension <= 1500 every eigenvalue lies within its residual
+    tolerance of the oracle's (numpy eigvalsh of the oracle's dense H: a Ritz value with
+    residual r is within r of an eigenvalue)."""
     reqs, rng = synth.random_requests(gpu, 24, 23, p_sd=0.7, max_sd=6, max_fp=4)
     ks = [int(min(rng.integers(1, 11), r[5].dim)) for r in reqs]
     tols = [float(10 ** rng.uniform(-6, -4)) for _ in reqs]
@@ -240,7 +243,7 @@ def test_serving_poisson_sd_fp_mixed_stream(gpu):
     res = srv.drain()
     srv.close()
     byid = {x["id"]: x for x in res}
-    overlapped = 0
+    overlapped = checked = 0
     for i, r, k, tol in zip(ids, reqs, ks, tols):
         x = byid[i]
         assert x["status"] == 0
@@ -254,4 +257,10 @@ def test_serving_poisson_sd_fp_mixed_stream(gpu):
         op.close()
         assert np.array_equal(x["evals"], e32), "served result differs from the request solved alone"
         assert np.max(np.abs(x["evals"] - e64)) <= 1e-3
+        if s.dim <= 1500:
+            o = oracle.System(r[4])
+            w = np.linalg.eigvalsh(oracle.dense_from_upper(s.dim, *o.h_upper(basis=s.basis())))
+            assert np.max(np.abs(x["evals"][:k] - w[:k])) <= tol, (s.dim, x["evals"][:k], w[:k])
+            checked += 1
     assert overlapped >= len(ids) // 2  # requests really ran beside one another
+    assert checked >= 5  # requests checked against the oracle
